@@ -1,0 +1,153 @@
+/* pelec_amd.h -- C-ABI of the MI355X (gfx950) cell-wise stiff-chemistry reactor.
+ *
+ * One shared library is built per mechanism (libpelec_amd_<mechanism>.so), the way the reference
+ * compiles one PelePhysics mechanism per build (Chemistry_Model, Exec/Make.PeleC; mechanism sources
+ * at CMake/BuildPelePhysicsLib.cmake:113-116). Every library exports the same symbols.
+ *
+ * The reference interface replaced here is pele::physics::reactions::ReactorBase as PeleC uses it:
+ *   create/init/close         Source/PeleC.cpp:2032-2047 (owner Source/PeleC.H:700-702)
+ *   set_typ_vals_ode          Source/React.cpp:7-27
+ *   react(box, ...)           Source/React.cpp:222-228
+ * (the ReactorBase header itself lives in the absent PelePhysics submodule).
+ *
+ * Conventions
+ *  - All array pointers are device pointers (HBM) unless stated; all calls that take a stream
+ *    enqueue work on it and return without synchronising (stream-ordered like the AMReX GPU
+ *    launch region the reference runs in; PeleC synchronises right after, React.cpp:230).
+ *  - Units are CGS (g, cm, s, K, erg), as PeleC and PelePhysics use them.
+ *  - Return value: PRC_OK (0) or a negative PRC_E* error code; per-cell integration failures are
+ *    reported through the optional per-cell diagnostics buffer, never by aborting the batch.
+ *  - A cell with mask == 0 is not integrated: its rY, T, rE are left bit-identical and FC = 0.
+ */
+#ifndef PELEC_AMD_H
+#define PELEC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRC_OK 0
+#define PRC_EINVAL (-1)     /* bad argument (null handle, unknown name/key, bad shape) */
+#define PRC_EHIP (-2)       /* a HIP runtime call failed */
+#define PRC_ESTATE (-3)     /* call not valid in the reactor's current state */
+#define PRC_ENOMEM (-4)     /* device allocation failed */
+
+/* Per-cell diagnostics record (int32, component-major, PRC_DIAG_NCOMP components):
+ * status, steps, RHS evaluations, Jacobian evaluations, LU factorisations, Newton iterations,
+ * error-test failures, convergence failures. status: 0 ok, 1 max steps, 2 repeated error-test
+ * failures, 3 repeated convergence failures, 4 bad input (rho <= 0, T <= 0 or dt <= 0). */
+#define PRC_DIAG_NCOMP 8
+
+/* Array4-shaped view of caller-owned device memory, mirroring amrex::Array4<Real>:
+ * element (i,j,k,n) is p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride];
+ * end[] is exclusive. */
+typedef struct prc_array4 {
+  double* p;
+  int64_t jstride, kstride, nstride;
+  int32_t begin[3];
+  int32_t end[3];
+  int32_t ncomp;
+} prc_array4;
+
+/* amrex::Array4<int> counterpart (the mask argument). */
+typedef struct prc_iarray4 {
+  int32_t* p;
+  int64_t jstride, kstride, nstride;
+  int32_t begin[3];
+  int32_t end[3];
+  int32_t ncomp;
+} prc_iarray4;
+
+typedef struct prc_reactor prc_reactor;
+
+/* ---- mechanism (compile-time, one per library) ------------------------------------------ */
+/* NUM_SPECIES of the mechanism (mechanism.H NUM_SPECIES). */
+int prc_num_species(void);
+/* Number of reactions (mechanism.H NUM_REACTIONS). */
+int prc_num_reactions(void);
+/* Name of species k in mechanism order (CKSYMS / the order pinned by the PMF files). */
+const char* prc_species_name(int k);
+/* Mechanism name and SHA-256 of the mechanism data file the library was generated from. */
+const char* prc_mechanism_name(void);
+const char* prc_mechanism_sha256(void);
+
+/* ---- reactor life cycle (ReactorBase) ---------------------------------------------------- */
+/* ReactorBase::create(chem_integrator), Source/PeleC.cpp:2034. Names: "ReactorCvode" (BDF),
+ * "ReactorRK64" (explicit RK with error control) and "ReactorNull" (no-op). */
+int prc_create(const char* chem_integrator, prc_reactor** out);
+/* ReactorBase::init(reactor_type, ncells), Source/PeleC.cpp:2040. Only reactor_type 1
+ * (constant-volume rho*e) exists in PeleC; ncells is a sizing hint. */
+int prc_init(prc_reactor* r, int reactor_type, int ncells);
+/* ReactorBase::set_typ_vals_ode(typ), Source/React.cpp:11,25: n = NUM_SPECIES + 1 typical values
+ * (rhoY_k ..., T); absolute tolerances become atol * typ_i. Pass n = 0 to clear. */
+int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
+/* ParmParse knobs the reactors read (ode.rtol, ode.atol, ode.max_steps, cvode.max_order) and the
+ * gfx950 execution form: "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched
+ * persistent work-queue kernel; "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
+int prc_set_param_real(prc_reactor* r, const char* key, double value);
+int prc_set_param_int(prc_reactor* r, const char* key, int value);
+/* Optional per-cell diagnostics (device int32 buffer, PRC_DIAG_NCOMP x capacity, component
+ * stride `capacity`); cell c of the next react call writes diag[comp*capacity + c]. Pass NULL to
+ * disable. */
+int prc_set_diagnostics(prc_reactor* r, int32_t* diag, int64_t capacity);
+/* ReactorBase::close(), Source/PeleC.cpp:2046; releases device workspace. */
+int prc_close(prc_reactor* r);
+int prc_destroy(prc_reactor* r);
+
+/* ---- reaction step ---------------------------------------------------------------------- */
+/* ReactorBase::react(box, rY, rYsrc_ext, T, rEner, rEner_src_ext, FC, mask, dt_react, time,
+ * stream), Source/React.cpp:222-228. Integrates every unmasked cell of the box [lo, hi]
+ * (inclusive) over dt_react:
+ *   d(rhoY_k)/dt = W_k wdot_k + rYsrc_k,  rho*e(t) = rEner + t * rEner_src.
+ * rY (NUM_SPECIES comps), T and rEner are updated in place; FC receives the RHS evaluation count.
+ * rYsrc_ext.p / rEner_src_ext.p / mask.p may be NULL (zero sources, all cells active). On return
+ * *time has advanced by *dt_react. */
+int prc_react_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3], prc_array4 rY,
+                  prc_array4 rYsrc_ext, prc_array4 T, prc_array4 rEner,
+                  prc_array4 rEner_src_ext, prc_array4 FC, prc_iarray4 mask, double* dt_react,
+                  double* time, void* stream);
+
+/* Flat (1-D) variant over ncells cells stored component-major: rY[k*ncells + c],
+ * rYsrc[k*ncells + c]; T, rE, rEsrc, FC, mask have one entry per cell. NULL rYsrc/rEsrc/mask/FC
+ * allowed. */
+int prc_react_flat(prc_reactor* r, double* rY, const double* rYsrc, double* T, double* rE,
+                   const double* rEsrc, double* FC, const int32_t* mask, int64_t ncells,
+                   double* dt_react, double* time, void* stream);
+
+/* Batched multi-box call: all boxes are integrated by one launch (the batched form walks one
+ * global cell queue across boxes). Equivalent to calling prc_react_box per box. Per-cell
+ * diagnostics are indexed by the cell's position in the concatenation of the boxes. */
+typedef struct prc_box_args {
+  int32_t lo[3], hi[3];
+  prc_array4 rY, rYsrc_ext, T, rEner, rEner_src_ext, FC;
+  prc_iarray4 mask;
+} prc_box_args;
+int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, double* dt_react,
+                    double* time, void* stream);
+
+/* Synchronise the stream and return the number of cells whose integration failed in the
+ * reactor's previous react call on it (status != 0). */
+int64_t prc_last_failures(prc_reactor* r, void* stream);
+
+/* ---- EOS helpers at the boundary (Fuego EOS calls PeleC makes around the reactor) -------- */
+/* PYT2RE: rho and e from p, T, Y (Exec/RegTests/zeroD/prob.cpp:34-39). Y component-major. */
+int prc_eos_pyt2re(int64_t ncells, const double* p, const double* T, const double* Y,
+                   double* rho, double* e, void* stream);
+/* REY2T: T from rho, e, Y by Newton, T in/out as initial guess (pc_cmpTemp, Source/Utilities.H:
+ * 68-85). */
+int prc_eos_rey2t(int64_t ncells, const double* rho, const double* e, const double* Y, double* T,
+                  void* stream);
+/* RTY2Hi: species enthalpies h_k(T) [erg/g] (heat release, Source/React.cpp:337-338); hi
+ * component-major. */
+int prc_eos_rty2hi(int64_t ncells, const double* T, double* hi, void* stream);
+/* Production rates (mechanism productionRate / RTY2WDOT): wdot_k [mol/(cm^3 s)] from rho, T, Y. */
+int prc_eos_rty2wdot(int64_t ncells, const double* rho, const double* T, const double* Y,
+                     double* wdot, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PELEC_AMD_H */

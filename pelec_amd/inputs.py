@@ -1,0 +1,143 @@
+"""Seeded synthetic reactor inputs for the workloads this project measures.
+
+Inputs are produced on the host from a counter-based generator (splitmix64 of seed, stream and
+cell index), so any subset of cells can be regenerated bit-identically and the GPU and the CPU
+baseline always consume the same states. Cell states follow Exec/RegTests/zeroD/prob.cpp:34-39
+(PYT2RE: rho from p, T, Y; rho*e = rho * sum_k Y_k e_k(T)) with zero external sources, as
+PeleC's react_state passes them on the first call (React.cpp:59-62).
+
+Workloads (species order of the mechanism files):
+  lidryer_32: LiDryer H2/air, 32^3 cells, T ~ U[1000,1600] K, p = 1 atm, phi ~ U[0.5,2];
+              H, O, OH, HO2 seeded at Y = 1e-6 then renormalised.
+  drm19_64:   drm19 CH4/air, 64^3 cells, T ~ U[900,2200] K, p ~ U[1,10] atm, phi ~ U[0.6,1.6];
+              10 % of cells ignition-seeded: T ~ U[1400,1600] K with Y_H = Y_OH = 1e-4.
+  drm19_adj:  drm19 CH4/air, 16384 cells, T ~ U[1000,1600] K, p = 1 atm, phi ~ U[0.5,2].
+Air is O2 + 3.76 N2 (no Ar); H2/air uses n_H2 = 2 phi, CH4/air n_CH4 = phi / 2 per O2.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import mechanism as M
+
+ATM = 1013250.0
+_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & _MASK64
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _MASK64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _MASK64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, idx: np.ndarray) -> np.ndarray:
+    """U[0,1) doubles, one per cell index, independent per (seed, stream)."""
+    with np.errstate(over="ignore"):
+        key = _splitmix64(np.uint64(seed) * np.uint64(0x100000001B3) + np.uint64(stream))
+        z = _splitmix64(idx.astype(np.uint64) ^ key)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+class Thermo:
+    """Host NASA-7 evaluation used to build the inputs (PYT2RE)."""
+
+    def __init__(self, mech: M.Mechanism):
+        self.W = np.array(mech.weights)
+        self.low = np.array([s.low for s in mech.species])
+        self.high = np.array([s.high for s in mech.species])
+        self.Tmid = np.array([s.Tmid for s in mech.species])
+
+    def e_mass(self, T: np.ndarray) -> np.ndarray:
+        """(ncells, N) mass-specific internal energies."""
+        T = T[:, None]
+        a = np.where((T < self.Tmid)[..., None], self.low, self.high)
+        hRT = (a[..., 0] + T * (a[..., 1] / 2 + T * (a[..., 2] / 3 + T * (a[..., 3] / 4 + T * a[..., 4] / 5)))
+               + a[..., 5] / T)
+        return (hRT - 1.0) * M.R_ERG * T / self.W
+
+    def pyt2re(self, p, T, Y):
+        Wbar = 1.0 / (Y / self.W).sum(-1)
+        rho = p * Wbar / (M.R_ERG * T)
+        e = (Y * self.e_mass(T)).sum(-1)
+        return rho, e
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    mech: str
+    ncells: int
+    dt: float
+    rtol: float
+    atol: float
+    seed: int
+
+
+WORKLOADS: Dict[str, Workload] = {
+    "lidryer_32": Workload("lidryer_32", "LiDryer", 32 ** 3, 1e-6, 1e-10, 1e-10, 1001),
+    "drm19_64": Workload("drm19_64", "drm19", 64 ** 3, 1e-5, 1e-6, 1e-10, 2002),
+    "drm19_adj": Workload("drm19_adj", "drm19", 16384, 1e-4, 1e-6, 1e-10, 4004),
+}
+
+
+def _fuel_air_Y(mech: M.Mechanism, fuel: str, nfuel: np.ndarray) -> np.ndarray:
+    N = mech.nsp
+    X = np.zeros((nfuel.shape[0], N))
+    X[:, mech.species_index(fuel)] = nfuel
+    X[:, mech.species_index("O2")] = 1.0
+    X[:, mech.species_index("N2")] = 3.76
+    W = np.array(mech.weights)
+    Y = X * W
+    return Y / Y.sum(-1, keepdims=True)
+
+
+def make_inputs(workload: str, ncells: Optional[int] = None, first: int = 0) -> Dict[str, np.ndarray]:
+    """Cells [first, first+ncells) of a workload. Returns rY (N, n), T, rE, p, phi, ign."""
+    wl = WORKLOADS[workload]
+    n = wl.ncells if ncells is None else ncells
+    mech = M.load(wl.mech)
+    th = Thermo(mech)
+    idx = np.arange(first, first + n, dtype=np.int64)
+    u = lambda s: uniform(wl.seed, s, idx)
+    ign = np.zeros(n, dtype=bool)
+    if wl.name == "lidryer_32":
+        T = 1000.0 + 600.0 * u(0)
+        p = np.full(n, ATM)
+        phi = 0.5 + 1.5 * u(2)
+        Y = _fuel_air_Y(mech, "H2", 2.0 * phi)
+        for sp in ("H", "O", "OH", "HO2"):
+            Y[:, mech.species_index(sp)] = 1e-6
+        Y /= Y.sum(-1, keepdims=True)
+    elif wl.name == "drm19_64":
+        T = 900.0 + 1300.0 * u(0)
+        p = ATM * (1.0 + 9.0 * u(1))
+        phi = 0.6 + 1.0 * u(2)
+        Y = _fuel_air_Y(mech, "CH4", 0.5 * phi)
+        ign = u(3) < 0.1
+        T = np.where(ign, 1400.0 + 200.0 * u(4), T)
+        for sp in ("H", "OH"):
+            Y[ign, mech.species_index(sp)] = 1e-4
+        Y /= Y.sum(-1, keepdims=True)
+    elif wl.name == "drm19_adj":
+        T = 1000.0 + 600.0 * u(0)
+        p = np.full(n, ATM)
+        phi = 0.5 + 1.5 * u(2)
+        Y = _fuel_air_Y(mech, "CH4", 0.5 * phi)
+    else:  # pragma: no cover
+        raise KeyError(workload)
+    rho, e = th.pyt2re(p, T, Y)
+    rY = np.ascontiguousarray((rho[:, None] * Y).T)
+    return {"rY": rY, "T": T.copy(), "rE": rho * e, "p": p, "phi": phi, "ign": ign,
+            "rho": rho, "Y": Y}
+
+
+def typical_values(rY: np.ndarray, T: np.ndarray, rhoY_min: float = 1e-10) -> np.ndarray:
+    """PeleC::set_typical_values_chem (Source/React.cpp:13-25): typ rhoY_k = max(mid-range,
+    typical_rhoY_val_min), typ T = mid-range of T."""
+    typ = np.maximum(0.5 * (rY.min(axis=1) + rY.max(axis=1)), rhoY_min)
+    return np.concatenate([typ, [0.5 * (T.min() + T.max())]])
