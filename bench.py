@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--sample", type=int, default=2048, help="cells checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["batched", "plain"], default=None,
+                    help="time one form only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -107,8 +109,16 @@ def main():
           for k in ("rY", "T", "rE")}
     n = wl.ncells
 
-    head = time_form(torch, Reactor, wl, d0, "batched", True, args.steps, args.warmup)
-    plain = time_form(torch, Reactor, wl, d0, "plain", False, args.steps, args.warmup)
+    head = plain = None
+    if args.only != "plain":
+        head = time_form(torch, Reactor, wl, d0, "batched", True, args.steps, args.warmup)
+    if args.only != "batched":
+        plain = time_form(torch, Reactor, wl, d0, "plain", False, args.steps, args.warmup)
+    if args.only:
+        o = head or plain
+        print(json.dumps({"form": args.only, "ms_per_step": float(np.median(o["times"])) * 1e3,
+                          "cells_per_s": n / float(np.median(o["times"]))}), flush=True)
+        return
 
     t_head = float(np.median(head["times"]))
     t_plain = float(np.median(plain["times"]))

@@ -1,0 +1,178 @@
+// Batched form of the reactor: cells of uneven stiffness integrated as a device-side work queue of
+// per-cell BDF phases, one cell per lane, every kernel running one uniform code path.
+//
+// Work is split by phase so a wave never executes another cell's rare branch:
+//   k_start   inputs -> state, f(t0,y0), initial step (cvHin), first attempt      (once per cell)
+//   k_jac     analytic Jacobian of the cells in the set-up list that need one       (~2 per cell)
+//   k_lu      Newton matrix I - gamma J factorised in LDS, one matrix per lane        (~10 per cell)
+//   k_newton  one Newton correction for every cell in the Newton phase: RHS, LU solve, convergence
+//             test, and -- when the correction converged -- the error test, step completion,
+//             order/step choice and the next step's prediction                      (~60 per cell)
+//   k_compact repacks live cells into a dense slot range when half the slots are done.
+// Cells that need a Newton-matrix set-up are appended (wave-aggregated) to a double-buffered list
+// consumed by k_jac/k_lu of the next iteration. The host replays a captured graph of a fixed
+// number of iterations and stops when every started cell is done.
+//
+// Storage: per-slot structure-of-arrays for vectors and scalars (coalesced: a wave's lanes hold
+// consecutive slots); per-cell array-of-structures for the n x n Jacobian and LU factors, so the
+// scattered cells of a set-up batch write whole contiguous blocks and the Newton lanes stream
+// their own block.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bdf_core.h"
+
+#ifndef PELEC_MEMORY_FENCE
+#define PELEC_MEMORY_FENCE() __asm__ __volatile__("" ::: "memory")
+#endif
+
+namespace pelec {
+
+enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2 };
+
+// device counters
+enum Ctr : int {
+  CT_SETUP0 = 0,   // length of set-up list 0
+  CT_SETUP1 = 1,   // length of set-up list 1
+  CT_STARTED = 2,  // cells whose integration started
+  CT_DONE = 3,     // started cells finished (ok or failed)
+  CT_FAILED = 4,   // cells with status != 0 (incl. bad input)
+  CT_NACTIVE = 5,  // slots in use [0, nactive)
+  CT_CUR = 6,      // state buffer holding the live slots (0/1)
+  CT_NEWACTIVE = 7,
+  CT_COUNT = 8
+};
+
+struct StateBuf {
+  double* zn;       // (QMAX+1) * n * C
+  double* acor;     // n * C
+  double* ewt;      // n * C
+  uint64_t* sc;     // BDF_SCALAR_WORDS * C
+  double* rhoe;     // 2 * C  (rho*e0, rho*e source)
+  double* src;      // N * C  (rhoY sources) or null
+  int32_t* cell;    // C  slot -> global cell index
+  int32_t* phase;   // C
+};
+
+template <int n>
+struct SoAV {
+  double* zn;
+  double* ac;
+  double* ew;
+  int64_t C;
+  int64_t s;
+  __device__ double& Z(int j, int i) const { return zn[((int64_t)j * n + i) * C + s]; }
+  __device__ double& A(int i) const { return ac[(int64_t)i * C + s]; }
+  __device__ double& E(int i) const { return ew[(int64_t)i * C + s]; }
+};
+
+__device__ inline void load_scalars(BdfScalars& S, const uint64_t* sc, int64_t C, int64_t s) {
+  uint64_t w[BDF_SCALAR_WORDS];
+#pragma unroll
+  for (int k = 0; k < BDF_SCALAR_WORDS; ++k) w[k] = sc[k * C + s];
+  __builtin_memcpy(&S, w, sizeof(S));
+}
+__device__ inline void store_scalars(const BdfScalars& S, uint64_t* sc, int64_t C, int64_t s) {
+  uint64_t w[BDF_SCALAR_WORDS];
+  __builtin_memcpy(w, &S, sizeof(S));
+#pragma unroll
+  for (int k = 0; k < BDF_SCALAR_WORDS; ++k) sc[k * C + s] = w[k];
+}
+
+// Wave-aggregated append of `val` for the lanes with pred set: one atomic per wave, lane order
+// preserved inside the wave.
+__device__ inline void wave_append(int32_t* list, unsigned long long* cnt, bool pred, int32_t val) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = (int)atomicAdd(cnt, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = val;
+}
+
+__device__ inline void wave_count(unsigned long long* cnt, bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  if (lane == __ffsll((long long)mask) - 1) atomicAdd(cnt, (unsigned long long)__popcll(mask));
+}
+
+// Dense LU with partial pivoting of the lane's matrix held in LDS at a[(r*n + c) * LANES] (one
+// column of the LDS array per lane: conflict-free). perm receives the row permutation (row i of
+// the factors is original row perm[i]); the diagonal of U is stored inverted.
+template <int n, typename T, int LANES>
+__device__ inline int lu_lds(T* a, int* perm) {
+#pragma unroll
+  for (int r = 0; r < n; ++r) perm[r * LANES] = r;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    T amax = fabs(a[(k * n + k) * LANES]);
+    for (int i = k + 1; i < n; ++i) {
+      const T v = fabs(a[(i * n + k) * LANES]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    if (amax == T(0)) return k + 1;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) {
+        const T t = a[(k * n + j) * LANES];
+        a[(k * n + j) * LANES] = a[(p * n + j) * LANES];
+        a[(p * n + j) * LANES] = t;
+      }
+      const int t = perm[k * LANES];
+      perm[k * LANES] = perm[p * LANES];
+      perm[p * LANES] = t;
+    }
+    const T inv = T(1) / a[(k * n + k) * LANES];
+    a[(k * n + k) * LANES] = inv;
+    T rowk[n];
+#pragma unroll
+    for (int j = 0; j < n; ++j)
+      if (j > k) rowk[j] = a[(k * n + j) * LANES];
+    for (int i = k + 1; i < n; ++i) {
+      const T l = a[(i * n + k) * LANES] * inv;
+      a[(i * n + k) * LANES] = l;
+#pragma unroll
+      for (int j = 0; j < n; ++j)
+        if (j > k) a[(i * n + j) * LANES] -= l * rowk[j];
+    }
+  }
+  return 0;
+}
+
+// x = (LU)^{-1} P b with factors from lu_lds (AoS block `lu`, inverted diagonal). b in/out (FP64),
+// arithmetic in T. `gb` is this lane's LDS scratch (stride 64) used to gather b by the permutation.
+template <int n, typename T>
+__device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b, double* gb) {
+#pragma unroll
+  for (int i = 0; i < n; ++i) gb[i * 64] = b[i];
+  T x[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) x[i] = T(gb[perm[i] * 64]);
+#pragma unroll
+  for (int i = 1; i < n; ++i) {
+    T s = x[i];
+#pragma unroll
+    for (int j = 0; j < n; ++j)
+      if (j < i) s -= lu[i * n + j] * x[j];
+    x[i] = s;
+    PELEC_MEMORY_FENCE();  // bound the loads in flight to one row
+  }
+#pragma unroll
+  for (int i = n - 1; i >= 0; --i) {
+    T s = x[i];
+#pragma unroll
+    for (int j = 0; j < n; ++j)
+      if (j > i) s -= lu[i * n + j] * x[j];
+    x[i] = s * lu[i * n + i];
+    PELEC_MEMORY_FENCE();
+  }
+#pragma unroll
+  for (int i = 0; i < n; ++i) b[i] = double(x[i]);
+}
+
+}  // namespace pelec

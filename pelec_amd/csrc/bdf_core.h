@@ -1,0 +1,493 @@
+// CVODE BDF step logic shared by the plain one-cell-per-lane kernel and the batched phase kernels.
+//
+// The integrator is CVODE's published variable-order (1..qmax) variable-step BDF in Nordsieck
+// form, as the reference runs it through ReactorCvode (Source/React.cpp:222-228; solver knobs
+// cvode.solve_type / cvode.max_order at Exec/Production/JetFlame/inputs.inp:30-31):
+//   cvSetBDF / cvSetTqBDF   -> set_bdf          cvPredict / cvRestore -> predict / restore
+//   cvRescale               -> rescale          cvIncreaseBDF / cvDecreaseBDF -> adjust_order
+//   cvCompleteStep          -> complete_step    cvPrepareNextStep / cvChooseEta / cvSetEta
+//   cvDoErrorTest           -> error_test_fail  cvHandleNFlag (conv. failure) -> conv_fail
+//   cvHin / cvUpperBoundH0 / cvYddNorm -> hin
+// with the step clipped to land exactly on t = dt_react (CVODE's tstop handling).
+//
+// Storage is abstracted: BdfScalars holds every per-cell scalar of the integrator, and a vector
+// accessor V provides Z(j,i) (Nordsieck column j), A(i) (accumulated correction) and E(i) (error
+// weights) as double references -- private arrays in the plain kernel, structure-of-arrays HBM in
+// the batched kernels. Both forms therefore take identical decisions for a cell.
+#pragma once
+#include "cell_model.h"
+
+namespace pelec {
+
+constexpr int BDF_QMAX = 5;
+
+enum CellStatus : int {
+  CELL_OK = 0,
+  CELL_MAXSTEPS = 1,
+  CELL_ERRFAIL = 2,
+  CELL_CONVFAIL = 3,
+  CELL_BADINPUT = 4,
+};
+
+struct BdfParams {
+  double rtol;
+  const double* atolv;  // n entries (device, shared by all cells)
+  int qmax;
+  int max_steps;
+};
+
+namespace cv {
+constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1,
+                 ETACF = 0.25, ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0,
+                 THRESH = 1.5, CRDOWN = 0.3, DGMAX = 0.3, RDIV = 2.0, NLSCOEF = 0.1,
+                 DGMAX_LS = 0.2, UROUND = 2.220446049250313e-16;
+constexpr int MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10, MAXCOR = 3, MXNCF = 10, MXNEF = 7,
+              MSBP = 20, MSBJ = 50, SMALL_NST = 10;
+}  // namespace cv
+
+// Every per-cell scalar of the integrator (kept as whole 8-byte words so the batched form can
+// stream it to and from structure-of-arrays storage).
+struct BdfScalars {
+  double tn, h, hscale, hprime, eta, etamax, saved_tq5;
+  double gamma, gammap, gamrat, crate, acnrm, delp, saved_t;
+  double tau[BDF_QMAX + 2], tq[6], l[BDF_QMAX + 1];
+  int q, qprime, qwait, L, nstlp, nstlj, indx_acor, jcur;
+  int nst, nfe, nje, nsetups, nni, netf, ncfn;
+  int m, convfail, nflag, ncf, nef;
+};
+static_assert(sizeof(BdfScalars) % 8 == 0, "BdfScalars must be whole words");
+constexpr int BDF_SCALAR_WORDS = sizeof(BdfScalars) / 8;
+
+__device__ inline void scalars_reset(BdfScalars& S, int qmax) {
+  S.tn = S.h = S.hscale = S.hprime = 0.0;
+  S.eta = 1.0;
+  S.etamax = cv::ETAMX1;
+  S.saved_tq5 = 0.0;
+  S.gamma = S.gammap = 0.0;
+  S.gamrat = S.crate = 1.0;
+  S.acnrm = S.delp = S.saved_t = 0.0;
+#pragma unroll
+  for (int i = 0; i < BDF_QMAX + 2; ++i) S.tau[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) S.tq[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < BDF_QMAX + 1; ++i) S.l[i] = 0.0;
+  S.q = 1;
+  S.qprime = 1;
+  S.L = 2;
+  S.qwait = 2;
+  S.nstlp = S.nstlj = 0;
+  S.indx_acor = qmax;
+  S.jcur = 0;
+  S.nst = S.nfe = S.nje = S.nsetups = S.nni = S.netf = S.ncfn = 0;
+  S.m = S.convfail = S.nflag = S.ncf = S.nef = 0;
+}
+
+template <int n, class V>
+__device__ inline double wrms_acor(const V& v) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    const double w = v.A(i) * v.E(i);
+    s += w * w;
+  }
+  return sqrt(s / n);
+}
+
+template <int n, class V>
+__device__ inline double wrms_reg(const V& v, const double* x) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    const double w = x[i] * v.E(i);
+    s += w * w;
+  }
+  return sqrt(s / n);
+}
+
+template <int n, class V>
+__device__ inline void set_ewt(const V& v, const BdfParams& prm) {
+#pragma unroll
+  for (int i = 0; i < n; ++i) v.E(i) = 1.0 / (prm.rtol * fabs(v.Z(0, i)) + prm.atolv[i]);
+}
+
+template <int n, class V>
+__device__ inline void rescale(BdfScalars& S, const V& v) {
+  double factor = S.eta;
+  for (int j = 1; j <= S.q; ++j) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) v.Z(j, i) *= factor;
+    factor *= S.eta;
+  }
+  S.h = S.hscale * S.eta;
+  S.hscale = S.h;
+}
+
+template <int n, class V>
+__device__ inline void predict(BdfScalars& S, const V& v, double tstop) {
+  S.tn += S.h;
+  if ((S.tn - tstop) * S.h > 0.0) S.tn = tstop;
+  for (int k = 1; k <= S.q; ++k)
+    for (int j = S.q; j >= k; --j)
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j - 1, i) += v.Z(j, i);
+}
+
+template <int n, class V>
+__device__ inline void restore(BdfScalars& S, const V& v) {
+  S.tn = S.saved_t;
+  for (int k = 1; k <= S.q; ++k)
+    for (int j = S.q; j >= k; --j)
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j - 1, i) -= v.Z(j, i);
+}
+
+__device__ inline void set_bdf(BdfScalars& S) {
+  double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+  const int q = S.q;
+  S.l[0] = S.l[1] = xi_inv = xistar_inv = 1.0;
+  for (int i = 2; i <= q; ++i) S.l[i] = 0.0;
+  alpha0 = alpha0_hat = -1.0;
+  hsum = S.h;
+  if (q > 1) {
+    for (int j = 2; j < q; ++j) {
+      hsum += S.tau[j - 1];
+      xi_inv = S.h / hsum;
+      alpha0 -= 1.0 / j;
+      for (int i = j; i >= 1; --i) S.l[i] += S.l[i - 1] * xi_inv;
+    }
+    alpha0 -= 1.0 / q;
+    xistar_inv = -S.l[1] - alpha0;
+    hsum += S.tau[q - 1];
+    xi_inv = S.h / hsum;
+    alpha0_hat = -S.l[1] - xi_inv;
+    for (int i = q; i >= 1; --i) S.l[i] += S.l[i - 1] * xistar_inv;
+  }
+  const double A1 = 1.0 - alpha0_hat + alpha0;
+  const double A2 = 1.0 + q * A1;
+  S.tq[2] = fabs(A1 / (alpha0 * A2));
+  S.tq[5] = fabs(A2 * xistar_inv / (S.l[q] * xi_inv));
+  if (S.qwait == 1) {
+    if (q > 1) {
+      const double Cc = xistar_inv / S.l[q];
+      const double A3 = alpha0 + 1.0 / q;
+      const double A4 = alpha0_hat + xi_inv;
+      const double Cpinv = (1.0 - A4 + A3) / A3;
+      S.tq[1] = fabs(Cc * Cpinv);
+    } else {
+      S.tq[1] = 1.0;
+    }
+    hsum += S.tau[q];
+    xi_inv = S.h / hsum;
+    const double A5 = alpha0 - 1.0 / (q + 1);
+    const double A6 = alpha0_hat - xi_inv;
+    const double Cppinv = (1.0 - A6 + A5) / A2;
+    S.tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+  }
+  S.tq[4] = cv::NLSCOEF / S.tq[2];
+}
+
+template <int n, class V>
+__device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax) {
+  if (S.q == 2 && dq != 1) return;
+  if (dq == 1) {
+    for (int i = 0; i <= qmax; ++i) S.l[i] = 0.0;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = S.hscale;
+    S.l[2] = 1.0;
+    if (S.q > 1) {
+      for (int j = 1; j < S.q; ++j) {
+        hsum += S.tau[j + 1];
+        const double xi = hsum / S.hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+        for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xiold + S.l[i - 1];
+        xiold = xi;
+      }
+    }
+    const double A1 = (-alpha0 - alpha1) / prod;
+#pragma unroll
+    for (int i = 0; i < n; ++i) v.Z(S.L, i) = A1 * v.Z(S.indx_acor, i);
+    for (int j = 2; j <= S.q; ++j)
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.Z(S.L, i);
+  } else {
+    for (int i = 0; i <= qmax; ++i) S.l[i] = 0.0;
+    S.l[2] = 1.0;
+    double hsum = 0.0;
+    for (int j = 1; j <= S.q - 2; ++j) {
+      hsum += S.tau[j];
+      const double xi = hsum / S.hscale;
+      for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xi + S.l[i - 1];
+    }
+    for (int j = 2; j < S.q; ++j)
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j, i) -= S.l[j] * v.Z(S.q, i);
+  }
+}
+
+// Start of cvStep: order/step change decided at the end of the previous step.
+template <int n, class V>
+__device__ inline void step_prologue(BdfScalars& S, const V& v, int qmax) {
+  S.saved_t = S.tn;
+  S.ncf = S.nef = 0;
+  S.nflag = 0;
+  if (S.nst > 0 && S.hprime != S.h) {
+    if (S.qprime != S.q) {
+      adjust_order<n>(S, v, S.qprime - S.q, qmax);
+      S.q = S.qprime;
+      S.L = S.q + 1;
+      S.qwait = S.L;
+    }
+    rescale<n>(S, v);
+  }
+}
+
+// Predict, compute BDF coefficients and gamma, decide whether the Newton matrix must be
+// (re)built (cvNls). Returns true when a set-up is needed; S.convfail is its argument.
+template <int n, class V>
+__device__ inline bool begin_attempt(BdfScalars& S, const V& v, double tstop) {
+  predict<n>(S, v, tstop);
+  set_bdf(S);
+  S.gamma = S.h / S.l[1];
+  if (S.nst == 0) S.gammap = S.gamma;
+  S.gamrat = (S.nst > 0) ? S.gamma / S.gammap : 1.0;
+  S.convfail = (S.nflag == 0 || S.nflag == 2) ? 0 : 1;
+  S.m = 0;
+  return (S.nflag == 1) || (S.nflag == 2) || (S.nst == 0) || (S.nst >= S.nstlp + cv::MSBP) ||
+         (fabs(S.gamrat - 1.0) > cv::DGMAX);
+}
+
+// Jacobian re-evaluation decision of the dense linear solver set-up (cvDlsSetup).
+__device__ inline bool jac_needed(const BdfScalars& S) {
+  const double dgamma = fabs(S.gamma / S.gammap - 1.0);
+  return (S.nst == 0) || (S.nst > S.nstlj + cv::MSBJ) ||
+         (S.convfail == 2 && dgamma < cv::DGMAX_LS) || (S.convfail == 1);
+}
+
+// Bookkeeping after the Newton matrix was rebuilt.
+__device__ inline void after_setup(BdfScalars& S) {
+  S.nsetups++;
+  S.gamrat = S.crate = 1.0;
+  S.gammap = S.gamma;
+  S.nstlp = S.nst;
+}
+
+// Unrecoverable-in-this-attempt Newton failure (cvHandleNFlag): returns CELL_CONVFAIL or -1 when
+// the step is retried with h * ETACF.
+template <int n, class V>
+__device__ inline int conv_fail(BdfScalars& S, const V& v) {
+  S.ncfn++;
+  restore<n>(S, v);
+  S.ncf++;
+  S.etamax = 1.0;
+  if (S.ncf == cv::MXNCF) return CELL_CONVFAIL;
+  S.eta = cv::ETACF;
+  S.nflag = 1;
+  rescale<n>(S, v);
+  return -1;
+}
+
+// Failed local error test (cvDoErrorTest). Returns CELL_ERRFAIL, -1 (retry), or -2 (retry after
+// an order-1 restart: the caller must set zn[1] = h * f(tn, zn[0])).
+template <int n, class V>
+__device__ inline int error_test_fail(BdfScalars& S, const V& v, double dsm, int qmax) {
+  S.nef++;
+  S.netf++;
+  S.nflag = 2;
+  restore<n>(S, v);
+  if (S.nef == cv::MXNEF) return CELL_ERRFAIL;
+  S.etamax = 1.0;
+  if (S.nef <= cv::MXNEF1) {
+    S.eta = 1.0 / (pow(cv::BIAS2 * dsm, 1.0 / S.L) + cv::ADDON);
+    S.eta = fmax(cv::ETAMIN, S.eta);
+    if (S.nef >= cv::SMALL_NEF) S.eta = fmin(S.eta, cv::ETAMXF);
+    rescale<n>(S, v);
+    return -1;
+  }
+  if (S.q > 1) {
+    S.eta = cv::ETAMIN;
+    adjust_order<n>(S, v, -1, qmax);
+    S.L = S.q;
+    S.q--;
+    S.qwait = S.L;
+    rescale<n>(S, v);
+    return -1;
+  }
+  S.eta = cv::ETAMIN;
+  S.h *= S.eta;
+  S.hscale = S.h;
+  S.qwait = cv::LONG_WAIT;
+  return -2;
+}
+
+__device__ inline void set_eta(BdfScalars& S) {
+  if (S.eta < cv::THRESH) {
+    S.eta = 1.0;
+    S.hprime = S.h;
+  } else {
+    S.eta = fmin(S.eta, S.etamax);
+    S.hprime = S.h * S.eta;
+  }
+}
+
+// cvCompleteStep + cvPrepareNextStep + etamax update after a successful step.
+template <int n, class V>
+__device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qmax) {
+  S.nst++;
+  for (int i = S.q; i >= 2; --i) S.tau[i] = S.tau[i - 1];
+  if (S.q == 1 && S.nst > 1) S.tau[2] = S.tau[1];
+  S.tau[1] = S.h;
+  for (int j = 0; j <= S.q; ++j)
+#pragma unroll
+    for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.A(i);
+  S.qwait--;
+  if (S.qwait == 1 && S.q != qmax) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) v.Z(qmax, i) = v.A(i);
+    S.saved_tq5 = S.tq[5];
+    S.indx_acor = qmax;
+  }
+  // cvPrepareNextStep
+  if (S.etamax == 1.0) {
+    S.qwait = S.qwait > 2 ? S.qwait : 2;
+    S.qprime = S.q;
+    S.hprime = S.h;
+    S.eta = 1.0;
+  } else {
+    const double etaq = 1.0 / (pow(cv::BIAS2 * dsm, 1.0 / S.L) + cv::ADDON);
+    if (S.qwait != 0) {
+      S.eta = etaq;
+      S.qprime = S.q;
+      set_eta(S);
+    } else {
+      S.qwait = 2;
+      double etaqm1 = 0.0;
+      if (S.q > 1) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+          const double w = v.Z(S.q, i) * v.E(i);
+          s += w * w;
+        }
+        const double ddn = sqrt(s / n) * S.tq[1];
+        etaqm1 = 1.0 / (pow(cv::BIAS1 * ddn, 1.0 / S.q) + cv::ADDON);
+      }
+      double etaqp1 = 0.0;
+      if (S.q != qmax && S.saved_tq5 != 0.0) {
+        const double cquot = (S.tq[5] / S.saved_tq5) * pow(S.h / S.tau[2], (double)S.L);
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+          const double w = (v.A(i) - cquot * v.Z(qmax, i)) * v.E(i);
+          s += w * w;
+        }
+        const double dup = sqrt(s / n) * S.tq[3];
+        etaqp1 = 1.0 / (pow(cv::BIAS3 * dup, 1.0 / (S.L + 1)) + cv::ADDON);
+      }
+      const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+      if (etam < cv::THRESH) {
+        S.eta = 1.0;
+        S.qprime = S.q;
+      } else if (etam == etaq) {
+        S.eta = etaq;
+        S.qprime = S.q;
+      } else if (etam == etaqm1) {
+        S.eta = etaqm1;
+        S.qprime = S.q - 1;
+      } else {
+        S.eta = etaqp1;
+        S.qprime = S.q + 1;
+#pragma unroll
+        for (int i = 0; i < n; ++i) v.Z(qmax, i) = v.A(i);
+      }
+      set_eta(S);
+    }
+  }
+  S.etamax = (S.nst <= cv::SMALL_NST) ? cv::ETAMX2 : cv::ETAMX3;
+}
+
+// After a successful step: true when tn has reached tstop; otherwise clips the next step so it
+// does not pass tstop.
+__device__ inline bool reached_tstop(BdfScalars& S, double tstop) {
+  const double troundoff = 100.0 * cv::UROUND * (fabs(S.tn) + fabs(S.h));
+  if (fabs(S.tn - tstop) <= troundoff) return true;
+  if ((S.tn + S.hprime - tstop) * S.h > 0.0) {
+    S.hprime = (tstop - S.tn) * (1.0 - 4.0 * cv::UROUND);
+    S.eta = S.hprime / S.h;
+  }
+  return false;
+}
+
+// cvHin: initial step size from an estimate of ||y''||; zn[1] holds f(t0, y0) on entry.
+// f(t, y, out) evaluates the RHS (and counts it).
+template <int n, class V, class F>
+__device__ inline double hin(BdfScalars& S, const V& v, double tout, F&& f) {
+  const double tdist = tout - S.tn;
+  const double tround = cv::UROUND * fmax(fabs(S.tn), fabs(tout));
+  const double hlb = 100.0 * tround;
+  double hub_inv = 0.0;
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    const double d = 0.1 * fabs(v.Z(0, i)) + 1.0 / v.E(i);
+    hub_inv = fmax(hub_inv, fabs(v.Z(1, i)) / d);
+  }
+  double hub = 0.1 * tdist;
+  if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+  double hg = sqrt(hlb * hub);
+  if (hub < hlb) return hg;
+  double hnew = hg;
+  double y[n], t[n];
+  for (int count1 = 1; count1 <= 4; ++count1) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) y[i] = v.Z(0, i) + hg * v.Z(1, i);
+    f(S.tn + hg, y, t);
+#pragma unroll
+    for (int i = 0; i < n; ++i) t[i] = (t[i] - v.Z(1, i)) / hg;
+    const double yddnrm = wrms_reg<n>(v, t);
+    hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+    if (count1 == 4) break;
+    const double hrat = hnew / hg;
+    if (hrat > 0.5 && hrat < 2.0) break;
+    if (count1 > 1 && hrat > 2.0) {
+      hnew = hg;
+      break;
+    }
+    hg = hnew;
+  }
+  return fmin(fmax(0.5 * hnew, hlb), hub);
+}
+
+// One Newton correction (after f(tn, y) is in ftemp): tempv = M^{-1}(gamma f - rl1 zn1 - acor),
+// scaled by 2/(1+gamrat); acor += tempv. Returns 0 converged, 1 continue, 2 failed (retry with a
+// fresh Jacobian when it is stale), 3 failed with a current Jacobian. The solve is a callable.
+template <int n, class V, class Solve>
+__device__ inline int newton_correct(BdfScalars& S, const V& v, const double* ftemp, Solve&& solve) {
+  double tempv[n];
+  const double rl1 = 1.0 / S.l[1];
+#pragma unroll
+  for (int i = 0; i < n; ++i) tempv[i] = S.gamma * ftemp[i] - (rl1 * v.Z(1, i) + v.A(i));
+  solve(tempv);
+  if (S.gamrat != 1.0) {
+    const double sc = 2.0 / (1.0 + S.gamrat);
+#pragma unroll
+    for (int i = 0; i < n; ++i) tempv[i] *= sc;
+  }
+  S.nni++;
+  const double del = wrms_reg<n>(v, tempv);
+#pragma unroll
+  for (int i = 0; i < n; ++i) v.A(i) += tempv[i];
+  if (S.m > 0) S.crate = fmax(cv::CRDOWN * S.crate, del / S.delp);
+  const double dcon = del * fmin(1.0, S.crate) / S.tq[4];
+  if (dcon <= 1.0) {
+    S.acnrm = (S.m == 0) ? del : wrms_acor<n>(v);
+    S.jcur = 0;
+    return 0;
+  }
+  S.m++;
+  if (S.m == cv::MAXCOR || (S.m >= 2 && del > cv::RDIV * S.delp)) return S.jcur ? 3 : 2;
+  S.delp = del;
+  return 1;
+}
+
+}  // namespace pelec

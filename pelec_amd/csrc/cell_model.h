@@ -13,6 +13,10 @@
 #include <cmath>
 #include <cstdint>
 
+#ifndef PELEC_MEMORY_FENCE
+#define PELEC_MEMORY_FENCE() __asm__ __volatile__("" ::: "memory")
+#endif
+
 namespace pelec {
 
 template <class Mech>
@@ -120,6 +124,7 @@ struct CellModel {
       }
       const double dfTdx = (dNdx - fT * cvk[m]) * iD;
       J[N * n + m] = JT(dfTdx + dfTdT * tau);
+      PELEC_MEMORY_FENCE();  // one column at a time
     }
 #pragma unroll
     for (int k = 0; k < n; ++k) J[k * n + N] = JT(0);

@@ -1,51 +1,68 @@
-// gfx950 reactor library: kernels and the C-ABI of include/pelec_amd.h for one mechanism.
+// gfx950 reactor library for one mechanism: host side and the C-ABI of include/pelec_amd.h.
 // Built once per mechanism with -DPELEC_MECH=<ident> -DPELEC_MECH_HEADER="gen/mech_<name>.h".
 //
-// Kernels
-//   react_plain    one cell per lane over a static grid (the plain FP64 drop-in for
-//                  ReactorCvode::react, Source/React.cpp:222-228); per-lane working set
-//                  (Nordsieck array, Jacobian, LU) in private memory.
-//   react_queue    persistent grid, one cell per lane, cells taken from a device work queue:
-//                  a lane that finishes its cell writes it back and pulls the next cell index,
-//                  so cells with 50x different step counts no longer hold a wave hostage.
-//   eos_*          boundary EOS helpers (PYT2RE, REY2T, RTY2Hi, RTY2WDOT).
+// prc.form 0 (plain): one launch of react_plain per box; prc.form 1 (batched): k_start, then a
+// captured graph of ITERS_PER_GRAPH iterations (k_jac, k_lu, k_newton) replayed until every
+// started cell is done, with a compaction of the live slots whenever fewer than half remain
+// alive. prc.lu_fp32 keeps the Newton-matrix factors in FP32 with FP64 refinement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
 
-#include "bdf_cell.h"
 #include "pelec_amd.h"
 #include PELEC_MECH_HEADER
 
 using Mech = pelec::mech::PELEC_MECH;
-using namespace pelec;
+
+#include "reactor_kernels.h"
+
+namespace {
+constexpr int ITERS_PER_GRAPH = 16;  // even: every replay starts on set-up list 0
+}
+
+struct prc_reactor {
+  std::string name;
+  int integrator = 0;  // 0 ReactorCvode (BDF), 2 ReactorNull
+  bool initialised = false;
+  double rtol = 1e-6, atol = 1e-10;
+  int max_steps = 100000;
+  int max_order = 5;
+  int form = 0;
+  int lu_fp32 = 0;
+  double typ[NV];
+  bool have_typ = false;
+  double* d_atol = nullptr;
+  unsigned long long* d_ctr = nullptr;
+  unsigned long long* h_ctr = nullptr;  // pinned
+  Args* d_args = nullptr;
+  BoxDev* d_boxes = nullptr;
+  int d_boxes_cap = 0;
+  int32_t* diag = nullptr;
+  int64_t diag_cap = 0;
+  // batched workspace
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int64_t ws_cap = 0;
+  int ws_lu_fp32 = -1;
+  int ws_src = -1;
+  Args ws_layout{};
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_lu_fp32 = -1;
+  int ncu = 0;
+  int occ_newton = 1, occ_jac = 1, occ_lu = 1;
+  int64_t last_iterations = 0;
+};
 
 namespace {
 
-constexpr int NSP = Mech::NSP;
-constexpr int NV = NSP + 1;
-
-// device view of an Array4
-struct A4 {
-  double* p;
-  int64_t js, ks, ns;
-  int b0, b1, b2;
-  __device__ double& at(int i, int j, int k, int c) const {
-    return p[(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks + (int64_t)c * ns];
-  }
-};
-struct A4I {
-  int32_t* p;
-  int64_t js, ks, ns;
-  int b0, b1, b2;
-  __device__ int32_t at(int i, int j, int k) const {
-    return p[(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks];
-  }
-};
+#define HIP_OK(expr)                           \
+  do {                                         \
+    if ((expr) != hipSuccess) return PRC_EHIP; \
+  } while (0)
 
 A4 mk(const prc_array4& a) {
   return A4{a.p, a.jstride, a.kstride, a.nstride, a.begin[0], a.begin[1], a.begin[2]};
@@ -53,270 +70,6 @@ A4 mk(const prc_array4& a) {
 A4I mki(const prc_iarray4& a) {
   return A4I{a.p, a.jstride, a.kstride, a.nstride, a.begin[0], a.begin[1], a.begin[2]};
 }
-
-struct BoxDev {
-  int lo0, lo1, lo2, len0, len1;
-  int64_t ncells;
-  int64_t first;  // global index of the box's first cell (batched concatenation)
-  A4 rY, rYsrc, T, rE, rEsrc, FC;
-  A4I mask;
-};
-
-struct LaunchArgs {
-  const BoxDev* boxes;  // device array (batched) or nullptr
-  BoxDev box;           // single box (plain)
-  int nboxes;
-  int64_t ncells_total;
-  double dt;
-  BdfParams prm;
-  int32_t* diag;
-  int64_t diag_cap;
-  unsigned long long* queue;     // work-queue head (batched)
-  unsigned long long* nfailed;   // failure counter
-};
-
-__device__ inline void cell_ijk(const BoxDev& b, int64_t c, int& i, int& j, int& k) {
-  i = b.lo0 + (int)(c % b.len0);
-  const int64_t r = c / b.len0;
-  j = b.lo1 + (int)(r % b.len1);
-  k = b.lo2 + (int)(r / b.len1);
-}
-
-__device__ inline void write_diag(const LaunchArgs& a, int64_t gidx, int status, int nst, int nfe,
-                                  int nje, int nsetups, int nni, int netf, int ncfn) {
-  if (!a.diag || gidx >= a.diag_cap) return;
-  int32_t* d = a.diag + gidx;
-  const int64_t s = a.diag_cap;
-  d[0] = status;
-  d[s] = nst;
-  d[2 * s] = nfe;
-  d[3 * s] = nje;
-  d[4 * s] = nsetups;
-  d[5 * s] = nni;
-  d[6 * s] = netf;
-  d[7 * s] = ncfn;
-}
-
-// Load cell (i,j,k) of box b into the integrator and start it. Returns false on bad input.
-template <typename LUT>
-__device__ inline bool load_start(BdfCell<Mech, LUT>& cell, const BoxDev& b, int i, int j, int k,
-                                  double dt, const BdfParams& prm) {
-  double y0[NV];
-  double rho = 0.0;
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) {
-    y0[s] = b.rY.at(i, j, k, s);
-    rho += y0[s];
-  }
-  y0[NSP] = b.T.at(i, j, k, 0);
-  cell.rhoe0 = b.rE.at(i, j, k, 0);
-  cell.rhoesrc = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
-  cell.src = b.rYsrc.p ? &b.rYsrc.at(i, j, k, 0) : nullptr;
-  cell.sstride = b.rYsrc.ns;
-  cell.prm = prm;
-  if (!(rho > 0.0) || !(y0[NSP] > 0.0) || !(dt > 0.0)) return false;
-  cell.start(y0, dt);
-  return true;
-}
-
-// Write the integrated state back: rhoY, T = T(e_end, Y) with the integrated T as guess,
-// rho*e = rho*e0 + dt * rho*e_src.
-template <typename LUT>
-__device__ inline void store(const BdfCell<Mech, LUT>& cell, const BoxDev& b, int i, int j, int k,
-                             double dt) {
-  double rho = 0.0;
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) rho += cell.zn[0][s];
-  const double irho = 1.0 / rho;
-  double Y[NSP];
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) Y[s] = cell.zn[0][s] * irho;
-  const double rEn = cell.rhoe0 + dt * cell.rhoesrc;
-  const double Tn = CellModel<Mech>::T_from_e(rEn * irho, Y, cell.zn[0][NSP]);
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) b.rY.at(i, j, k, s) = cell.zn[0][s];
-  b.T.at(i, j, k, 0) = Tn;
-  b.rE.at(i, j, k, 0) = rEn;
-}
-
-// ------------------------------------------------------------------ plain FP64 drop-in
-template <typename LUT>
-__global__ void __launch_bounds__(64) react_plain(LaunchArgs a) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const BoxDev& b = a.box;
-  if (c >= b.ncells) return;
-  int i, j, k;
-  cell_ijk(b, c, i, j, k);
-  const int64_t g = b.first + c;
-  if (b.mask.p && b.mask.at(i, j, k) == 0) {
-    if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
-    write_diag(a, g, CELL_OK, 0, 0, 0, 0, 0, 0, 0);
-    return;
-  }
-  BdfCell<Mech, LUT> cell;
-  int s;
-  if (!load_start(cell, b, i, j, k, a.dt, a.prm)) {
-    s = CELL_BADINPUT;
-    cell.nst = cell.nfe = cell.nje = cell.nsetups = cell.nni = cell.netf = cell.ncfn = 0;
-  } else {
-    do {
-      s = cell.advance();
-    } while (s < 0);
-    if (s == CELL_OK) store(cell, b, i, j, k, a.dt);
-  }
-  if (s != CELL_OK) atomicAdd(a.nfailed, 1ull);
-  if (b.FC.p) b.FC.at(i, j, k, 0) = (double)cell.nfe;
-  write_diag(a, g, s, cell.nst, cell.nfe, cell.nje, cell.nsetups, cell.nni, cell.netf, cell.ncfn);
-}
-
-// ------------------------------------------------------------------ batched work-queue form
-__device__ inline int find_box(const LaunchArgs& a, int64_t g) {
-  int lo = 0, hi = a.nboxes - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.boxes[mid].first <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-template <typename LUT>
-__global__ void __launch_bounds__(64) react_queue(LaunchArgs a) {
-  BdfCell<Mech, LUT> cell;
-  int64_t g = -1;  // global cell index held by this lane
-  int bi = 0, ci = 0, cj = 0, ck = 0;
-  bool need = true;
-  for (;;) {
-    if (need) {
-      // pull the next unmasked cell; masked cells are completed on the way
-      for (;;) {
-        g = (int64_t)atomicAdd(a.queue, 1ull);
-        if (g >= a.ncells_total) {
-          g = -1;
-          break;
-        }
-        bi = a.boxes ? find_box(a, g) : 0;
-        const BoxDev& b = a.boxes ? a.boxes[bi] : a.box;
-        cell_ijk(b, g - b.first, ci, cj, ck);
-        if (b.mask.p && b.mask.at(ci, cj, ck) == 0) {
-          if (b.FC.p) b.FC.at(ci, cj, ck, 0) = 0.0;
-          write_diag(a, g, CELL_OK, 0, 0, 0, 0, 0, 0, 0);
-          continue;
-        }
-        if (!load_start(cell, b, ci, cj, ck, a.dt, a.prm)) {
-          if (b.FC.p) b.FC.at(ci, cj, ck, 0) = 0.0;
-          write_diag(a, g, CELL_BADINPUT, 0, 0, 0, 0, 0, 0, 0);
-          atomicAdd(a.nfailed, 1ull);
-          continue;
-        }
-        break;
-      }
-      need = false;
-    }
-    if (g < 0) break;  // queue drained for this lane
-    const int s = cell.advance();
-    if (s >= 0) {
-      const BoxDev& b = a.boxes ? a.boxes[bi] : a.box;
-      if (s == CELL_OK)
-        store(cell, b, ci, cj, ck, a.dt);
-      else
-        atomicAdd(a.nfailed, 1ull);
-      if (b.FC.p) b.FC.at(ci, cj, ck, 0) = (double)cell.nfe;
-      write_diag(a, g, s, cell.nst, cell.nfe, cell.nje, cell.nsetups, cell.nni, cell.netf,
-                 cell.ncfn);
-      need = true;
-    }
-  }
-}
-
-// counters are cleared by a kernel rather than a memset node so the launch can be captured
-__global__ void zero_counters(unsigned long long* c) {
-  if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
-}
-
-// ------------------------------------------------------------------ EOS helper kernels
-__global__ void eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y,
-                           double* rho, double* e) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  double ek[NSP], cv[NSP];
-  Mech::e_cv(T[c], ek, cv);
-  double iW = 0.0, es = 0.0;
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) {
-    const double y = Y[s * nc + c];
-    iW += y * Mech::IW[s];
-    es += y * ek[s];
-  }
-  rho[c] = p[c] / (8.31446261815324e7 * T[c] * iW);
-  e[c] = es;
-}
-
-__global__ void eos_rey2t(int64_t nc, const double* rho, const double* e, const double* Y,
-                          double* T) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  double y[NSP];
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) y[s] = Y[s * nc + c];
-  (void)rho;
-  T[c] = CellModel<Mech>::T_from_e(e[c], y, T[c]);
-}
-
-__global__ void eos_rty2hi(int64_t nc, const double* T, double* hi) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  double ek[NSP], cv[NSP];
-  const double t = T[c];
-  Mech::e_cv(t, ek, cv);
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) hi[s * nc + c] = ek[s] + 8.31446261815324e7 * t * Mech::IW[s];
-}
-
-__global__ void eos_rty2wdot(int64_t nc, const double* rho, const double* T, const double* Y,
-                             double* wdot) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  double C[NSP], wd[NSP];
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) C[s] = rho[c] * Y[s * nc + c] * Mech::IW[s];
-  Mech::wdot(C, T[c], wd);
-#pragma unroll
-  for (int s = 0; s < NSP; ++s) wdot[s * nc + c] = wd[s];
-}
-
-}  // namespace
-
-// ====================================================================== host side
-struct prc_reactor {
-  std::string name;
-  int integrator = 0;  // 0 ReactorCvode (BDF), 1 ReactorRK64, 2 ReactorNull
-  int reactor_type = 1;
-  bool initialised = false;
-  double rtol = 1e-6, atol = 1e-10;
-  int max_steps = 100000;
-  int max_order = 5;
-  int form = 0;      // 0 plain, 1 batched
-  int lu_fp32 = 0;   // Newton-matrix LU precision
-  double typ[NV];
-  bool have_typ = false;
-  double* d_atol = nullptr;
-  unsigned long long* d_counters = nullptr;  // [0] queue head, [1] failures
-  BoxDev* d_boxes = nullptr;
-  int d_boxes_cap = 0;
-  int32_t* diag = nullptr;
-  int64_t diag_cap = 0;
-  int device = -1;
-};
-
-namespace {
-
-#define HIP_OK(expr)                      \
-  do {                                    \
-    if ((expr) != hipSuccess) return PRC_EHIP; \
-  } while (0)
 
 int upload_atol(prc_reactor* r) {
   double h[NV];
@@ -348,46 +101,199 @@ BoxDev make_box(const int32_t lo[3], const int32_t hi[3], const prc_array4& rY,
   return b;
 }
 
-bool box_ok(const BoxDev& b) {
-  return b.rY.p && b.T.p && b.rE.p && b.len0 >= 0 && b.len1 >= 0;
+bool box_ok(const BoxDev& b) { return b.ncells == 0 || (b.rY.p && b.T.p && b.rE.p); }
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Batched workspace for C cells: two slot-state buffers, per-cell J / LU / perm, lists, remap.
+int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
+  const int want_fp32 = r->lu_fp32;
+  if (r->ws && C <= r->ws_cap && r->ws_lu_fp32 == want_fp32 && r->ws_src >= (int)has_src)
+    return PRC_OK;
+  if (r->ws) {
+    (void)hipFree(r->ws);
+    r->ws = nullptr;
+    r->ws_cap = 0;
+  }
+  const int64_t cap = std::max<int64_t>(C, 64);
+  const size_t lub = want_fp32 ? sizeof(float) : sizeof(double);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align_up(bytes);
+    return o;
+  };
+  size_t o_st[2][8];
+  for (int b = 0; b < 2; ++b) {
+    o_st[b][0] = take(sizeof(double) * (BDF_QMAX + 1) * NV * cap);
+    o_st[b][1] = take(sizeof(double) * NV * cap);
+    o_st[b][2] = take(sizeof(double) * NV * cap);
+    o_st[b][3] = take(sizeof(uint64_t) * BDF_SCALAR_WORDS * cap);
+    o_st[b][4] = take(sizeof(double) * 2 * cap);
+    o_st[b][5] = has_src ? take(sizeof(double) * NSP * cap) : 0;
+    o_st[b][6] = take(sizeof(int32_t) * cap);
+    o_st[b][7] = take(sizeof(int32_t) * cap);
+  }
+  const size_t o_J = take(sizeof(double) * NV * NV * cap);
+  const size_t o_LU = take(lub * NV * NV * cap);
+  const size_t o_perm = take(sizeof(int32_t) * NV * cap);
+  const size_t o_l0 = take(sizeof(int32_t) * cap);
+  const size_t o_l1 = take(sizeof(int32_t) * cap);
+  const size_t o_remap = take(sizeof(int32_t) * cap);
+  void* p = nullptr;
+  if (hipMalloc(&p, off) != hipSuccess) return PRC_ENOMEM;
+  char* base = static_cast<char*>(p);
+  Args& L = r->ws_layout;
+  for (int b = 0; b < 2; ++b) {
+    L.st[b].zn = reinterpret_cast<double*>(base + o_st[b][0]);
+    L.st[b].acor = reinterpret_cast<double*>(base + o_st[b][1]);
+    L.st[b].ewt = reinterpret_cast<double*>(base + o_st[b][2]);
+    L.st[b].sc = reinterpret_cast<uint64_t*>(base + o_st[b][3]);
+    L.st[b].rhoe = reinterpret_cast<double*>(base + o_st[b][4]);
+    L.st[b].src = has_src ? reinterpret_cast<double*>(base + o_st[b][5]) : nullptr;
+    L.st[b].cell = reinterpret_cast<int32_t*>(base + o_st[b][6]);
+    L.st[b].phase = reinterpret_cast<int32_t*>(base + o_st[b][7]);
+  }
+  L.C = cap;
+  L.J = reinterpret_cast<double*>(base + o_J);
+  L.LU = base + o_LU;
+  L.perm = reinterpret_cast<int32_t*>(base + o_perm);
+  L.list[0] = reinterpret_cast<int32_t*>(base + o_l0);
+  L.list[1] = reinterpret_cast<int32_t*>(base + o_l1);
+  L.remap = reinterpret_cast<int32_t*>(base + o_remap);
+  r->ws = p;
+  r->ws_bytes = off;
+  r->ws_cap = cap;
+  r->ws_lu_fp32 = want_fp32;
+  r->ws_src = has_src;
+  return PRC_OK;
 }
 
-int launch(prc_reactor* r, LaunchArgs& a, hipStream_t s) {
+template <typename LUT>
+int capture_graph(prc_reactor* r) {
+  if (r->graph_exec && r->graph_lu_fp32 == r->lu_fp32) return PRC_OK;
+  if (r->graph_exec) {
+    (void)hipGraphExecDestroy(r->graph_exec);
+    r->graph_exec = nullptr;
+  }
+  hipStream_t s = r->cap_stream;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int gj = r->ncu * r->occ_jac;
+  const int gl = r->ncu * r->occ_lu;
+  const int gn = r->ncu * r->occ_newton;
+  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
+    const int par = it & 1;
+    hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(k_lu<LUT>, dim3(gl), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s, r->d_args, par);
+  }
+  hipGraph_t graph = nullptr;
+  if (hipStreamEndCapture(s, &graph) != hipSuccess) return PRC_EHIP;
+  const hipError_t e = hipGraphInstantiate(&r->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return PRC_EHIP;
+  r->graph_lu_fp32 = r->lu_fp32;
+  return PRC_OK;
+}
+
+template <typename LUT>
+int occupancy_setup(prc_reactor* r) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipDeviceGetAttribute(&r->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int o = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_newton<LUT>, 64, 0));
+  r->occ_newton = std::max(o, 1);
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_jac, 64, 0));
+  r->occ_jac = std::max(o, 1);
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_lu<LUT>, 64, 0));
+  r->occ_lu = std::max(o, 1);
+  return PRC_OK;
+}
+
+template <typename LUT>
+int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
+  if (r->graph_lu_fp32 != r->lu_fp32 || !r->ncu) {
+    const int e = occupancy_setup<LUT>(r);
+    if (e) return e;
+  }
+  HIP_OK(hipMemcpyAsync(r->d_args, &a, sizeof(Args), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, a.ncells);
+  const int64_t nb = (a.ncells + 63) / 64;
+  hipLaunchKernelGGL(k_start, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+  HIP_OK(hipGetLastError());
+  // the captured graph only reads r->d_args, so it is independent of the call
+  HIP_OK(hipStreamSynchronize(s));
+  int e = capture_graph<LUT>(r);
+  if (e) return e;
+  const int gn = r->ncu * r->occ_newton;
+  int64_t iters = 0;
+  // safety net: every cell ends within max_steps steps of a bounded number of attempts
+  const int64_t max_iters = (int64_t)(r->max_steps + 1) * 64 + 64;
+  int64_t nactive = a.ncells;
+  for (;;) {
+    HIP_OK(hipGraphLaunch(r->graph_exec, s));
+    iters += ITERS_PER_GRAPH;
+    HIP_OK(hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const int64_t started = (int64_t)r->h_ctr[CT_STARTED];
+    const int64_t done = (int64_t)r->h_ctr[CT_DONE];
+    if (done >= started) break;
+    if (iters > max_iters) return PRC_ESTATE;
+    const int64_t alive = started - done;
+    if (alive * 2 < nactive && nactive > 4096) {
+      hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, r->d_args);
+      hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, r->d_args, 0);
+      hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, r->d_args);
+      HIP_OK(hipGetLastError());
+      nactive = alive;
+    }
+  }
+  r->last_iterations = iters;
+  return PRC_OK;
+}
+
+// One react call: a.box (single box) or a.boxes / a.nboxes (device table), a.ncells cells.
+int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_counters = true) {
   a.prm.rtol = r->rtol;
   a.prm.atolv = r->d_atol;
   a.prm.qmax = std::min(std::max(r->max_order, 1), BDF_QMAX);
   a.prm.max_steps = r->max_steps;
   a.diag = r->diag;
   a.diag_cap = r->diag_cap;
-  a.queue = r->d_counters;
-  a.nfailed = r->d_counters + 1;
-  hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, s, r->d_counters);
-  if (a.ncells_total == 0) return PRC_OK;
-  if (r->form == 0 && a.boxes == nullptr) {
+  a.ctr = r->d_ctr;
+  if (a.ncells == 0 || r->form == 0) {
+    if (reset_counters) {
+      hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, (int64_t)0);
+      HIP_OK(hipGetLastError());
+    }
+    if (a.ncells == 0) return PRC_OK;
+    HIP_OK(hipMemcpyAsync(r->d_args, &a, sizeof(Args), hipMemcpyHostToDevice, s));
     const int64_t nb = (a.box.ncells + 63) / 64;
     if (r->lu_fp32)
-      hipLaunchKernelGGL(react_plain<float>, dim3((unsigned)nb), dim3(64), 0, s, a);
+      hipLaunchKernelGGL(react_plain<float>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
     else
-      hipLaunchKernelGGL(react_plain<double>, dim3((unsigned)nb), dim3(64), 0, s, a);
-  } else {
-    int dev = 0, ncu = 0;
-    HIP_OK(hipGetDevice(&dev));
-    HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu = 0;
-    if (r->lu_fp32)
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, react_queue<float>, 64, 0));
-    else
-      HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, react_queue<double>, 64, 0));
-    per_cu = std::max(per_cu, 1);
-    int64_t nb = (int64_t)ncu * per_cu;
-    nb = std::min<int64_t>(nb, (a.ncells_total + 63) / 64);
-    if (r->lu_fp32)
-      hipLaunchKernelGGL(react_queue<float>, dim3((unsigned)nb), dim3(64), 0, s, a);
-    else
-      hipLaunchKernelGGL(react_queue<double>, dim3((unsigned)nb), dim3(64), 0, s, a);
+      hipLaunchKernelGGL(react_plain<double>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+    HIP_OK(hipGetLastError());
+    // the argument block is reused by the next call: keep it until this launch has read it
+    HIP_OK(hipStreamSynchronize(s));
+    return PRC_OK;
   }
-  HIP_OK(hipGetLastError());
-  return PRC_OK;
+  int e = ensure_workspace(r, a.ncells, has_src);
+  if (e) return e;
+  const Args& L = r->ws_layout;
+  a.st[0] = L.st[0];
+  a.st[1] = L.st[1];
+  a.C = L.C;
+  a.J = L.J;
+  a.LU = L.LU;
+  a.perm = L.perm;
+  a.list[0] = L.list[0];
+  a.list[1] = L.list[1];
+  a.remap = L.remap;
+  a.has_src = has_src ? 1 : 0;
+  return r->lu_fp32 ? run_batched<float>(r, a, s) : run_batched<double>(r, a, s);
 }
 
 int ensure_init(prc_reactor* r) {
@@ -396,13 +302,38 @@ int ensure_init(prc_reactor* r) {
   return PRC_OK;
 }
 
-template <typename K, typename... Args>
-int launch_1d(K kern, int64_t nc, void* stream, Args... args) {
+template <typename K, typename... Xs>
+int launch_1d(K kern, int64_t nc, void* stream, Xs... xs) {
   if (nc <= 0) return PRC_OK;
   const int64_t nb = (nc + 255) / 256;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, nc, args...);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, nc, xs...);
   HIP_OK(hipGetLastError());
   return PRC_OK;
+}
+
+void release(prc_reactor* r) {
+  if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+  if (r->cap_stream) (void)hipStreamDestroy(r->cap_stream);
+  if (r->ws) (void)hipFree(r->ws);
+  if (r->d_atol) (void)hipFree(r->d_atol);
+  if (r->d_ctr) (void)hipFree(r->d_ctr);
+  if (r->h_ctr) (void)hipHostFree(r->h_ctr);
+  if (r->d_args) (void)hipFree(r->d_args);
+  if (r->d_boxes) (void)hipFree(r->d_boxes);
+  r->graph_exec = nullptr;
+  r->cap_stream = nullptr;
+  r->ws = nullptr;
+  r->ws_cap = 0;
+  r->ws_lu_fp32 = r->ws_src = -1;
+  r->d_atol = nullptr;
+  r->d_ctr = nullptr;
+  r->h_ctr = nullptr;
+  r->d_args = nullptr;
+  r->d_boxes = nullptr;
+  r->d_boxes_cap = 0;
+  r->graph_lu_fp32 = -1;
+  r->ncu = 0;
+  r->initialised = false;
 }
 
 }  // namespace
@@ -436,12 +367,19 @@ int prc_init(prc_reactor* r, int reactor_type, int ncells) {
   (void)ncells;
   if (!r || reactor_type != 1) return PRC_EINVAL;
   if (!r->initialised) {
-    HIP_OK(hipGetDevice(&r->device));
-    if (hipMalloc(&r->d_atol, NV * sizeof(double)) != hipSuccess) return PRC_ENOMEM;
-    if (hipMalloc(&r->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) return PRC_ENOMEM;
+    if (hipMalloc(&r->d_atol, NV * sizeof(double)) != hipSuccess ||
+        hipMalloc(&r->d_ctr, CT_COUNT * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&r->d_args, sizeof(Args)) != hipSuccess ||
+        hipHostMalloc(&r->h_ctr, CT_COUNT * sizeof(unsigned long long), 0) != hipSuccess) {
+      release(r);
+      return PRC_ENOMEM;
+    }
+    if (hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+      release(r);
+      return PRC_EHIP;
+    }
     r->initialised = true;
   }
-  r->reactor_type = reactor_type;
   return upload_atol(r);
 }
 
@@ -499,20 +437,13 @@ int prc_set_diagnostics(prc_reactor* r, int32_t* diag, int64_t capacity) {
 
 int prc_close(prc_reactor* r) {
   if (!r) return PRC_EINVAL;
-  if (r->d_atol) (void)hipFree(r->d_atol);
-  if (r->d_counters) (void)hipFree(r->d_counters);
-  if (r->d_boxes) (void)hipFree(r->d_boxes);
-  r->d_atol = nullptr;
-  r->d_counters = nullptr;
-  r->d_boxes = nullptr;
-  r->d_boxes_cap = 0;
-  r->initialised = false;
+  release(r);
   return PRC_OK;
 }
 
 int prc_destroy(prc_reactor* r) {
   if (!r) return PRC_EINVAL;
-  prc_close(r);
+  release(r);
   delete r;
   return PRC_OK;
 }
@@ -524,20 +455,18 @@ int prc_react_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3], prc_
   int e = ensure_init(r);
   if (e) return e;
   if (!lo || !hi || !dt_react || !time) return PRC_EINVAL;
-  LaunchArgs a{};
+  if (rY.ncomp != 0 && rY.ncomp < NSP) return PRC_EINVAL;
+  Args a{};
   a.box = make_box(lo, hi, rY, rYsrc, T, rE, rEsrc, FC, mask, 0);
   if (!box_ok(a.box)) return PRC_EINVAL;
-  if (rY.ncomp < NSP && rY.ncomp != 0) return PRC_EINVAL;
   a.boxes = nullptr;
   a.nboxes = 1;
-  a.ncells_total = a.box.ncells;
+  a.ncells = a.box.ncells;
   a.dt = *dt_react;
-  if (r->integrator == 2) {
-    *time += *dt_react;
-    return PRC_OK;
+  if (r->integrator != 2) {
+    e = launch(r, a, rYsrc.p != nullptr, (hipStream_t)stream);
+    if (e) return e;
   }
-  e = launch(r, a, (hipStream_t)stream);
-  if (e) return e;
   *time += *dt_react;
   return PRC_OK;
 }
@@ -545,18 +474,21 @@ int prc_react_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3], prc_
 int prc_react_flat(prc_reactor* r, double* rY, const double* rYsrc, double* T, double* rE,
                    const double* rEsrc, double* FC, const int32_t* mask, int64_t ncells,
                    double* dt_react, double* time, void* stream) {
-  if (ncells < 0 || ncells > INT32_MAX) return PRC_EINVAL;
+  int e = ensure_init(r);
+  if (e) return e;
+  if (ncells < 0 || ncells > INT32_MAX || !dt_react || !time) return PRC_EINVAL;
+  if (ncells == 0) {
+    *time += *dt_react;
+    return PRC_OK;
+  }
   const int32_t lo[3] = {0, 0, 0};
   const int32_t hi[3] = {(int32_t)ncells - 1, 0, 0};
   auto view = [&](const double* p, int nc) {
     prc_array4 v{};
     v.p = const_cast<double*>(p);
-    v.jstride = ncells;
-    v.kstride = ncells;
-    v.nstride = ncells;
+    v.jstride = v.kstride = v.nstride = ncells;
     v.end[0] = (int32_t)ncells;
-    v.end[1] = 1;
-    v.end[2] = 1;
+    v.end[1] = v.end[2] = 1;
     v.ncomp = nc;
     return v;
   };
@@ -566,11 +498,6 @@ int prc_react_flat(prc_reactor* r, double* rY, const double* rYsrc, double* T, d
   m.end[0] = (int32_t)ncells;
   m.end[1] = m.end[2] = 1;
   m.ncomp = 1;
-  if (ncells == 0) {
-    if (!r) return PRC_EINVAL;
-    if (dt_react && time) *time += *dt_react;
-    return ensure_init(r);
-  }
   return prc_react_box(r, lo, hi, view(rY, NSP), view(rYsrc, NSP), view(T, 1), view(rE, 1),
                        view(rEsrc, 1), view(FC, 1), m, dt_react, time, stream);
 }
@@ -590,29 +517,31 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
     int64_t first = 0;
     for (int ib = 0; ib < nboxes; ++ib) {
       const prc_box_args& x = boxes[ib];
-      LaunchArgs a{};
+      Args a{};
       a.box = make_box(x.lo, x.hi, x.rY, x.rYsrc_ext, x.T, x.rEner, x.rEner_src_ext, x.FC,
                        x.mask, first);
       if (!box_ok(a.box)) return PRC_EINVAL;
       a.nboxes = 1;
-      a.ncells_total = a.box.ncells;
+      a.ncells = a.box.ncells;
       a.dt = *dt_react;
       first += a.box.ncells;
-      if (a.box.ncells == 0) continue;
-      e = launch(r, a, s);
+      if (a.box.ncells == 0 && ib > 0) continue;
+      e = launch(r, a, x.rYsrc_ext.p != nullptr, s, ib == 0);
       if (e) return e;
     }
     *time += *dt_react;
     return PRC_OK;
   }
-  // batched form: one launch over the concatenation of all boxes
+  // batched form: one work queue over the concatenation of all boxes
   BoxDev* hb = new (std::nothrow) BoxDev[nboxes];
   if (!hb) return PRC_ENOMEM;
   int64_t first = 0;
+  bool has_src = false;
   for (int ib = 0; ib < nboxes; ++ib) {
     const prc_box_args& x = boxes[ib];
-    hb[ib] = make_box(x.lo, x.hi, x.rY, x.rYsrc_ext, x.T, x.rEner, x.rEner_src_ext, x.FC, x.mask,
-                      first);
+    hb[ib] = make_box(x.lo, x.hi, x.rY, x.rYsrc_ext, x.T, x.rEner, x.rEner_src_ext, x.FC,
+                      x.mask, first);
+    has_src = has_src || x.rYsrc_ext.p != nullptr;
     if (!box_ok(hb[ib])) {
       delete[] hb;
       return PRC_EINVAL;
@@ -620,7 +549,7 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
     first += hb[ib].ncells;
   }
   if (nboxes > r->d_boxes_cap) {
-    if (r->d_boxes) HIP_OK(hipFree(r->d_boxes));
+    if (r->d_boxes) (void)hipFree(r->d_boxes);
     r->d_boxes = nullptr;
     r->d_boxes_cap = 0;
     if (hipMalloc(&r->d_boxes, sizeof(BoxDev) * nboxes) != hipSuccess) {
@@ -629,23 +558,15 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
     }
     r->d_boxes_cap = nboxes;
   }
-  // the box table is read by the kernel: copy it in stream order, then release the host copy
-  if (hipMemcpyAsync(r->d_boxes, hb, sizeof(BoxDev) * nboxes, hipMemcpyHostToDevice, s) !=
-      hipSuccess) {
-    delete[] hb;
-    return PRC_EHIP;
-  }
-  if (hipStreamSynchronize(s) != hipSuccess) {
-    delete[] hb;
-    return PRC_EHIP;
-  }
+  const hipError_t ce = hipMemcpy(r->d_boxes, hb, sizeof(BoxDev) * nboxes, hipMemcpyHostToDevice);
   delete[] hb;
-  LaunchArgs a{};
+  if (ce != hipSuccess) return PRC_EHIP;
+  Args a{};
   a.boxes = r->d_boxes;
   a.nboxes = nboxes;
-  a.ncells_total = first;
+  a.ncells = first;
   a.dt = *dt_react;
-  e = launch(r, a, s);
+  e = launch(r, a, has_src, s);
   if (e) return e;
   *time += *dt_react;
   return PRC_OK;
@@ -653,12 +574,11 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
 
 int64_t prc_last_failures(prc_reactor* r, void* stream) {
   if (!r || !r->initialised) return PRC_EINVAL;
-  unsigned long long h = 0;
-  if (hipMemcpyAsync(&h, r->d_counters + 1, sizeof(h), hipMemcpyDeviceToHost,
-                     (hipStream_t)stream) != hipSuccess)
+  if (hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess)
     return PRC_EHIP;
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return PRC_EHIP;
-  return (int64_t)h;
+  return (int64_t)r->h_ctr[CT_FAILED];
 }
 
 int prc_eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y, double* rho,
@@ -668,8 +588,9 @@ int prc_eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y
 }
 int prc_eos_rey2t(int64_t nc, const double* rho, const double* e, const double* Y, double* T,
                   void* stream) {
+  (void)rho;
   if (nc < 0 || (nc > 0 && (!e || !Y || !T))) return PRC_EINVAL;
-  return launch_1d(eos_rey2t, nc, stream, rho, e, Y, T);
+  return launch_1d(eos_rey2t, nc, stream, e, Y, T);
 }
 int prc_eos_rty2hi(int64_t nc, const double* T, double* hi, void* stream) {
   if (nc < 0 || (nc > 0 && (!T || !hi))) return PRC_EINVAL;

@@ -1,0 +1,603 @@
+// Device kernels of the gfx950 reactor for one mechanism (included by reactor.hip, which defines
+// Mech). Forms:
+//   plain    react_plain: one cell per lane over a static grid, the whole integration of the cell
+//            in one call, working set in private memory -- the straightforward drop-in for
+//            ReactorCvode::react (Source/React.cpp:222-228) and the baseline.
+//   batched  device work queue of per-cell BDF phases (batched.h): k_start, then iterations of
+//            k_jac / k_lu / k_newton over live slots, compaction when half the slots are done.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "batched.h"
+#include "bdf_cell.h"
+
+namespace {
+
+using namespace pelec;
+
+constexpr int NSP = Mech::NSP;
+constexpr int NV = NSP + 1;
+using Model = CellModel<Mech>;
+
+// device view of an Array4
+struct A4 {
+  double* p;
+  int64_t js, ks, ns;
+  int b0, b1, b2;
+  __device__ double& at(int i, int j, int k, int c) const {
+    return p[(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks + (int64_t)c * ns];
+  }
+};
+struct A4I {
+  int32_t* p;
+  int64_t js, ks, ns;
+  int b0, b1, b2;
+  __device__ int32_t at(int i, int j, int k) const {
+    return p[(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks];
+  }
+};
+
+struct BoxDev {
+  int lo0, lo1, lo2, len0, len1;
+  int64_t ncells;
+  int64_t first;  // global index of the box's first cell (concatenation of the call's boxes)
+  A4 rY, rYsrc, T, rE, rEsrc, FC;
+  A4I mask;
+};
+
+// Everything a kernel needs for one react call, kept in device memory so the captured graph of
+// the batched form does not depend on the call.
+struct Args {
+  const BoxDev* boxes;  // device table (multi-box) or null
+  BoxDev box;           // single box
+  int nboxes;
+  int64_t ncells;       // cells of the call
+  double dt;
+  BdfParams prm;
+  int32_t* diag;
+  int64_t diag_cap;
+  unsigned long long* ctr;  // counters (CT_*)
+  // batched workspace
+  StateBuf st[2];
+  int64_t C;
+  double* J;       // per cell n*n (FP64)
+  void* LU;        // per cell n*n (LUT)
+  int32_t* perm;   // per cell n
+  int32_t* list[2];
+  int32_t* remap;
+  int has_src;
+};
+
+__device__ inline void cell_ijk(const BoxDev& b, int64_t c, int& i, int& j, int& k) {
+  i = b.lo0 + (int)(c % b.len0);
+  const int64_t r = c / b.len0;
+  j = b.lo1 + (int)(r % b.len1);
+  k = b.lo2 + (int)(r / b.len1);
+}
+
+__device__ inline const BoxDev& box_of(const Args& a, int64_t g, int64_t& local) {
+  if (!a.boxes) {
+    local = g - a.box.first;
+    return a.box;
+  }
+  int lo = 0, hi = a.nboxes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.boxes[mid].first <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  local = g - a.boxes[lo].first;
+  return a.boxes[lo];
+}
+
+__device__ inline void write_diag(const Args& a, int64_t g, int status, const BdfScalars* S) {
+  if (!a.diag || g >= a.diag_cap) return;
+  int32_t* d = a.diag + g;
+  const int64_t s = a.diag_cap;
+  d[0] = status;
+  d[s] = S ? S->nst : 0;
+  d[2 * s] = S ? S->nfe : 0;
+  d[3 * s] = S ? S->nje : 0;
+  d[4 * s] = S ? S->nsetups : 0;
+  d[5 * s] = S ? S->nni : 0;
+  d[6 * s] = S ? S->netf : 0;
+  d[7 * s] = S ? S->ncfn : 0;
+}
+
+// Final write-back of a cell: rhoY, T = T(e_end, Y) with the integrated T as guess, and
+// rho*e = rho*e0 + dt rho*e_src (Source/React.cpp:292 convention). Failed cells keep their inputs.
+template <class ZN>
+__device__ inline void store_cell(const Args& a, int64_t g, int status, const BdfScalars& S,
+                                  ZN zn0, double rhoe0, double rhoesrc) {
+  int64_t loc;
+  const BoxDev& b = box_of(a, g, loc);
+  int i, j, k;
+  cell_ijk(b, loc, i, j, k);
+  if (status == CELL_OK) {
+    double y[NV], Y[NSP];
+    double rho = 0.0;
+#pragma unroll
+    for (int s = 0; s < NV; ++s) y[s] = zn0(s);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) rho += y[s];
+    const double irho = 1.0 / rho;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) Y[s] = y[s] * irho;
+    const double rEn = rhoe0 + a.dt * rhoesrc;
+    const double Tn = Model::T_from_e(rEn * irho, Y, y[NSP]);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) b.rY.at(i, j, k, s) = y[s];
+    b.T.at(i, j, k, 0) = Tn;
+    b.rE.at(i, j, k, 0) = rEn;
+  }
+  if (b.FC.p) b.FC.at(i, j, k, 0) = (double)S.nfe;
+  write_diag(a, g, status, &S);
+}
+
+// ------------------------------------------------------------------ plain FP64 drop-in
+template <typename LUT>
+__global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
+  const Args& a = *ap;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const BoxDev& b = a.box;
+  if (c >= b.ncells) return;
+  int i, j, k;
+  cell_ijk(b, c, i, j, k);
+  const int64_t g = b.first + c;
+  if (b.mask.p && b.mask.at(i, j, k) == 0) {
+    if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+    write_diag(a, g, CELL_OK, nullptr);
+    return;
+  }
+  BdfCell<Mech, LUT> cell;
+  double y0[NV];
+  double rho = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) {
+    y0[s] = b.rY.at(i, j, k, s);
+    rho += y0[s];
+  }
+  y0[NSP] = b.T.at(i, j, k, 0);
+  cell.rhoe0 = b.rE.at(i, j, k, 0);
+  cell.rhoesrc = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
+  cell.src = b.rYsrc.p ? &b.rYsrc.at(i, j, k, 0) : nullptr;
+  cell.sstride = b.rYsrc.ns;
+  cell.prm = a.prm;
+  if (!(rho > 0.0) || !(y0[NSP] > 0.0) || !(a.dt > 0.0)) {
+    atomicAdd(&a.ctr[CT_FAILED], 1ull);
+    if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+    write_diag(a, g, CELL_BADINPUT, nullptr);
+    return;
+  }
+  cell.start(y0, a.dt);
+  int s;
+  do {
+    s = cell.advance();
+  } while (s < 0);
+  if (s != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
+  store_cell(a, g, s, cell.S, [&](int e) { return cell.zn[0][e]; }, cell.rhoe0, cell.rhoesrc);
+}
+
+// ------------------------------------------------------------------ batched work-queue form
+__device__ inline SoAV<NV> soa(const StateBuf& st, int64_t C, int64_t s) {
+  return SoAV<NV>{st.zn, st.acor, st.ewt, C, s};
+}
+
+__device__ inline void rhs_slot(const Args& a, const StateBuf& st, int64_t s, double t,
+                                const double* y, double* out) {
+  const double* src = a.has_src ? st.src + s : nullptr;
+  Model::rhs(t, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C, out);
+}
+
+// inputs -> state, f(t0, y0), cvHin, first attempt (always a set-up: nst == 0)
+__global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap) {
+  const Args& a = *ap;
+  const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const StateBuf& st = a.st[0];
+  bool started = false;
+  if (g < a.ncells) {
+    int64_t loc;
+    const BoxDev& b = box_of(a, g, loc);
+    int i, j, k;
+    cell_ijk(b, loc, i, j, k);
+    st.cell[g] = (int32_t)g;
+    st.phase[g] = PH_DONE;
+    if (b.mask.p && b.mask.at(i, j, k) == 0) {
+      if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+      write_diag(a, g, CELL_OK, nullptr);
+    } else {
+      double y0[NV];
+      double rho = 0.0;
+#pragma unroll
+      for (int s = 0; s < NSP; ++s) {
+        y0[s] = b.rY.at(i, j, k, s);
+        rho += y0[s];
+      }
+      y0[NSP] = b.T.at(i, j, k, 0);
+      if (!(rho > 0.0) || !(y0[NSP] > 0.0) || !(a.dt > 0.0)) {
+        atomicAdd(&a.ctr[CT_FAILED], 1ull);
+        if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+        write_diag(a, g, CELL_BADINPUT, nullptr);
+      } else {
+        started = true;
+        st.rhoe[g] = b.rE.at(i, j, k, 0);
+        st.rhoe[a.C + g] = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
+        if (a.has_src) {
+#pragma unroll
+          for (int s = 0; s < NSP; ++s)
+            st.src[s * a.C + g] = b.rYsrc.p ? b.rYsrc.at(i, j, k, s) : 0.0;
+        }
+        const SoAV<NV> v = soa(st, a.C, g);
+        BdfScalars S;
+        scalars_reset(S, a.prm.qmax);
+#pragma unroll
+        for (int e = 0; e < NV; ++e) v.Z(0, e) = y0[e];
+        set_ewt<NV>(v, a.prm);
+        auto f = [&](double t, const double* yy, double* out) {
+          S.nfe++;
+          rhs_slot(a, st, g, t, yy, out);
+        };
+        double f0[NV];
+        f(S.tn, y0, f0);
+#pragma unroll
+        for (int e = 0; e < NV; ++e) v.Z(1, e) = f0[e];
+        S.h = hin<NV>(S, v, a.dt, f);
+        if ((S.tn + S.h - a.dt) * S.h > 0.0) S.h = (a.dt - S.tn) * (1.0 - 4.0 * cv::UROUND);
+        S.hscale = S.h;
+        S.hprime = S.h;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) v.Z(1, e) *= S.h;
+        step_prologue<NV>(S, v, a.prm.qmax);
+        begin_attempt<NV>(S, v, a.dt);  // nst == 0: set-up required
+        store_scalars(S, st.sc, a.C, g);
+        st.phase[g] = PH_SETUP;
+      }
+    }
+  }
+  wave_count(&a.ctr[CT_STARTED], started);
+  wave_append(a.list[0], &a.ctr[CT_SETUP0], started, (int32_t)g);
+}
+
+// Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_jac(const Args* __restrict__ ap, int par) {
+  const Args& a = *ap;
+  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + threadIdx.x;
+    if (idx >= cnt) continue;
+    const int64_t s = a.list[par][idx];
+    BdfScalars S;
+    load_scalars(S, st.sc, a.C, s);
+    if (jac_needed(S)) {
+      S.nje++;
+      S.nstlj = S.nst;
+      S.jcur = 1;
+      const int64_t g = st.cell[s];
+      const SoAV<NV> v = soa(st, a.C, s);
+      double y[NV];
+#pragma unroll
+      for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e);
+      const double* src = a.has_src ? st.src + s : nullptr;
+      Model::template jac<double>(S.tn, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
+                                  a.J + g * (int64_t)(NV * NV));
+    } else {
+      S.jcur = 0;
+    }
+    store_scalars(S, st.sc, a.C, s);
+  }
+}
+
+// Newton matrices of the set-up list, factorised in LDS (one matrix per lane; FP64 factors use
+// half the lanes per pass so the LDS tile stays at 32 matrices)
+template <typename LUT>
+__global__ void __launch_bounds__(64) k_lu(const Args* __restrict__ ap, int par) {
+  constexpr int LANES = sizeof(LUT) == 4 ? 64 : 32;
+  __shared__ LUT lds[NV * NV * LANES];
+  __shared__ int plds[NV * LANES];
+  const Args& a = *ap;
+  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  const int lane = threadIdx.x;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
+    const int64_t idx = base + lane;
+    const bool valid = idx < cnt;
+    const int64_t s = valid ? a.list[par][idx] : 0;
+    const int64_t g = valid ? st.cell[s] : 0;
+    BdfScalars S;
+    if (valid) load_scalars(S, st.sc, a.C, s);
+    int ier = 0;
+    for (int pass = 0; pass < 64 / LANES; ++pass) {
+      if (valid && lane / LANES == pass) {
+        const int l = lane % LANES;
+        LUT* A = lds + l;
+        int* P = plds + l;
+        const double* Jg = a.J + g * (int64_t)(NV * NV);
+        for (int e = 0; e < NV * NV; ++e) A[e * LANES] = LUT(-S.gamma * Jg[e]);
+#pragma unroll
+        for (int d = 0; d < NV; ++d) A[(d * NV + d) * LANES] += LUT(1);
+        ier = lu_lds<NV, LUT, LANES>(A, P);
+        LUT* LUg = reinterpret_cast<LUT*>(a.LU) + g * (int64_t)(NV * NV);
+        for (int e = 0; e < NV * NV; ++e) LUg[e] = A[e * LANES];
+        int32_t* pg = a.perm + g * NV;
+#pragma unroll
+        for (int d = 0; d < NV; ++d) pg[d] = P[d * LANES];
+      }
+      __syncthreads();
+    }
+    bool requeue = false;
+    if (valid) {
+      const SoAV<NV> v = soa(st, a.C, s);
+      after_setup(S);
+      if (ier != 0) {
+        const int r = conv_fail<NV>(S, v);
+        if (r >= 0) {
+          st.phase[s] = PH_DONE;
+          atomicAdd(&a.ctr[CT_FAILED], 1ull);
+          atomicAdd(&a.ctr[CT_DONE], 1ull);
+          store_cell(a, g, r, S, [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
+        } else {
+          begin_attempt<NV>(S, v, a.dt);  // nflag = 1: set-up again
+          requeue = true;
+        }
+      } else {
+        st.phase[s] = PH_NEWTON;
+      }
+      store_scalars(S, st.sc, a.C, s);
+    }
+    wave_append(a.list[par ^ 1], &a.ctr[CT_SETUP0 + (par ^ 1)], requeue, (int32_t)s);
+  }
+}
+
+// One Newton correction per Newton-phase slot; on convergence the error test, step completion,
+// order/step choice and the next attempt's prediction.
+template <typename LUT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_newton(const Args* __restrict__ ap, int par) {
+  __shared__ double gb[NV * 64];
+  const Args& a = *ap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + par] = 0;  // consumed by k_jac/k_lu
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
+  const int lane = threadIdx.x;
+  constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
+  constexpr int W_M = offsetof(BdfScalars, m) / 8;
+  constexpr int SH_M = (offsetof(BdfScalars, m) % 8) * 8;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < nact; base += (int64_t)gridDim.x * 64) {
+    const int64_t s = base + lane;
+    const bool act = s < nact && st.phase[s] == PH_NEWTON;
+    bool need_setup = false, done = false;
+    if (act) {
+      const SoAV<NV> v = soa(st, a.C, s);
+      const int64_t g = st.cell[s];
+      // f(tn, zn[0] + acor), loading only tn and m first (keeps the RHS register budget)
+      double tn;
+      {
+        const uint64_t w = st.sc[W_TN * a.C + s];
+        __builtin_memcpy(&tn, &w, 8);
+      }
+      const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
+      double ftemp[NV];
+      {
+        double y[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e) + (m0 > 0 ? v.A(e) : 0.0);
+        rhs_slot(a, st, s, tn, y, ftemp);
+      }
+      BdfScalars S;
+      load_scalars(S, st.sc, a.C, s);
+      S.nfe++;
+      if (S.m == 0) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) v.A(e) = 0.0;
+      }
+      const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV);
+      const int32_t* pm = a.perm + g * NV;
+      const double* Jg = a.J + g * (int64_t)(NV * NV);
+      const double gammap = S.gammap;
+      double* gbl = gb + lane;
+      auto solve = [&](double* b) {
+        if constexpr (sizeof(LUT) == sizeof(double)) {
+          lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
+        } else {
+          double x[NV], r[NV];
+#pragma unroll
+          for (int e = 0; e < NV; ++e) x[e] = b[e];
+          lu_solve_perm<NV, LUT>(lu, pm, x, gbl);
+#pragma unroll
+          for (int e = 0; e < NV; ++e) {
+            double jx = 0.0;
+#pragma unroll
+            for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
+            r[e] = b[e] - (x[e] - gammap * jx);
+            PELEC_MEMORY_FENCE();
+          }
+          lu_solve_perm<NV, LUT>(lu, pm, r, gbl);
+#pragma unroll
+          for (int e = 0; e < NV; ++e) b[e] = x[e] + r[e];
+        }
+      };
+      const int r = newton_correct<NV>(S, v, ftemp, solve);
+      int status = -1;  // <0 running, else final status
+      if (r == 2) {
+        S.convfail = 2;  // stale Jacobian: rebuild and restart the Newton iteration
+        S.m = 0;
+        need_setup = true;
+      } else if (r != 1) {
+        bool attempt = false;
+        if (r == 3) {
+          const int e = conv_fail<NV>(S, v);
+          if (e >= 0)
+            status = e;
+          else
+            attempt = true;
+        } else {
+          const double dsm = S.acnrm * S.tq[2];
+          if (dsm <= 1.0) {
+            finish_step<NV>(S, v, dsm, a.prm.qmax);
+            if (reached_tstop(S, a.dt)) {
+              status = CELL_OK;
+            } else if (S.nst >= a.prm.max_steps) {
+              status = CELL_MAXSTEPS;
+            } else {
+              set_ewt<NV>(v, a.prm);
+              step_prologue<NV>(S, v, a.prm.qmax);
+              attempt = true;
+            }
+          } else {
+            const int e = error_test_fail<NV>(S, v, dsm, a.prm.qmax);
+            if (e >= 0) {
+              status = e;
+            } else {
+              if (e == -2) {
+                double y[NV], t[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) y[q] = v.Z(0, q);
+                S.nfe++;
+                rhs_slot(a, st, s, S.tn, y, t);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) v.Z(1, q) = S.h * t[q];
+              }
+              attempt = true;
+            }
+          }
+        }
+        if (attempt) need_setup = begin_attempt<NV>(S, v, a.dt);
+      }
+      if (status >= 0) {
+        done = true;
+        st.phase[s] = PH_DONE;
+        if (status != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
+        store_cell(a, g, status, S, [&](int e) { return v.Z(0, e); }, st.rhoe[s],
+                   st.rhoe[a.C + s]);
+      } else if (need_setup) {
+        st.phase[s] = PH_SETUP;
+      }
+      store_scalars(S, st.sc, a.C, s);
+    }
+    wave_append(a.list[par ^ 1], &a.ctr[CT_SETUP0 + (par ^ 1)], need_setup, (int32_t)s);
+    wave_count(&a.ctr[CT_DONE], done);
+  }
+}
+
+// Repack live slots (phase != DONE) densely into the other state buffer, lane order preserved.
+__global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap) {
+  const Args& a = *ap;
+  const int cur = (int)a.ctr[CT_CUR];
+  const StateBuf& src = a.st[cur];
+  const StateBuf& dst = a.st[cur ^ 1];
+  const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
+  const int lane = threadIdx.x;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < nact; base += (int64_t)gridDim.x * 64) {
+    const int64_t s = base + lane;
+    const bool alive = s < nact && src.phase[s] != PH_DONE;
+    const unsigned long long mask = __ballot(alive);
+    int nbase = 0;
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      if (lane == leader)
+        nbase = (int)atomicAdd(&a.ctr[CT_NEWACTIVE], (unsigned long long)__popcll(mask));
+      nbase = __shfl(nbase, leader, 64);
+    }
+    if (s < nact) a.remap[s] = -1;
+    if (alive) {
+      const int64_t d = nbase + __popcll(mask & ((1ull << lane) - 1ull));
+      a.remap[s] = (int32_t)d;
+      const int64_t C = a.C;
+      for (int e = 0; e < (BDF_QMAX + 1) * NV; ++e) dst.zn[e * C + d] = src.zn[e * C + s];
+#pragma unroll
+      for (int e = 0; e < NV; ++e) {
+        dst.acor[e * C + d] = src.acor[e * C + s];
+        dst.ewt[e * C + d] = src.ewt[e * C + s];
+      }
+#pragma unroll
+      for (int e = 0; e < BDF_SCALAR_WORDS; ++e) dst.sc[e * C + d] = src.sc[e * C + s];
+      dst.rhoe[d] = src.rhoe[s];
+      dst.rhoe[C + d] = src.rhoe[C + s];
+      if (a.has_src) {
+#pragma unroll
+        for (int e = 0; e < NSP; ++e) dst.src[e * C + d] = src.src[e * C + s];
+      }
+      dst.cell[d] = src.cell[s];
+      dst.phase[d] = src.phase[s];
+    }
+  }
+}
+
+// Renumber the pending set-up list after a compaction.
+__global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int par) {
+  const Args& a = *ap;
+  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  for (int64_t idx = (int64_t)blockIdx.x * 64 + threadIdx.x; idx < cnt;
+       idx += (int64_t)gridDim.x * 64)
+    a.list[par][idx] = a.remap[a.list[par][idx]];
+}
+
+__global__ void k_compact_done(const Args* __restrict__ ap) {
+  const Args& a = *ap;
+  if (threadIdx.x == 0) {
+    a.ctr[CT_CUR] ^= 1ull;
+    a.ctr[CT_NACTIVE] = a.ctr[CT_NEWACTIVE];
+    a.ctr[CT_NEWACTIVE] = 0ull;
+  }
+}
+
+// counters are cleared by a kernel rather than a memset node (graph-capturable)
+__global__ void k_init_counters(unsigned long long* c, int64_t nactive) {
+  if (threadIdx.x < CT_COUNT)
+    c[threadIdx.x] = (threadIdx.x == CT_NACTIVE) ? (unsigned long long)nactive : 0ull;
+}
+
+// ------------------------------------------------------------------ EOS helper kernels
+__global__ void eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y,
+                           double* rho, double* e) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double ek[NSP], cv[NSP];
+  Mech::e_cv(T[c], ek, cv);
+  double iW = 0.0, es = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) {
+    const double y = Y[s * nc + c];
+    iW += y * Mech::IW[s];
+    es += y * ek[s];
+  }
+  rho[c] = p[c] / (8.31446261815324e7 * T[c] * iW);
+  e[c] = es;
+}
+
+__global__ void eos_rey2t(int64_t nc, const double* e, const double* Y, double* T) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double y[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) y[s] = Y[s * nc + c];
+  T[c] = Model::T_from_e(e[c], y, T[c]);
+}
+
+__global__ void eos_rty2hi(int64_t nc, const double* T, double* hi) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double ek[NSP], cv[NSP];
+  const double t = T[c];
+  Mech::e_cv(t, ek, cv);
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) hi[s * nc + c] = ek[s] + 8.31446261815324e7 * t * Mech::IW[s];
+}
+
+__global__ void eos_rty2wdot(int64_t nc, const double* rho, const double* T, const double* Y,
+                             double* wdot) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double C[NSP], wd[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) C[s] = rho[c] * Y[s * nc + c] * Mech::IW[s];
+  Mech::wdot(C, T[c], wd);
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) wdot[s * nc + c] = wd[s];
+}
+
+}  // namespace

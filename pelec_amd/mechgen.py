@@ -7,14 +7,15 @@ The emitted header is written for gfx950 one-cell-per-lane execution:
 
 * every reaction is straight-line code with its constants folded in (no table lookups, so the
   per-lane arrays stay in VGPRs after unrolling);
-* reverse rates use per-species exp(g_k/RT) and its reciprocal, so a reversible reaction costs
-  products instead of an extra exp: 1/Kc = prod_k exp(g_k/RT)^nu_k * (P_atm/RT)^-dnu;
+* reverse rates use per-species exp(g_k/RT), so a reversible reaction costs products and one
+  division instead of an extra exp: 1/Kc = prod_k exp(g_k/RT)^nu_k * (P_atm/RT)^-dnu;
 * thermo is emitted as folded polynomials in T of e_k, cv_k, dcv_k/dT (mass units) and as
   per-range mixture-polynomial coefficient sums, so T(e, Y) Newton iterations cost one degree-5
   polynomial per iteration;
 * third-body concentrations are ctot + sum (alpha-1) C_k over the non-unit efficiencies only;
-* the Jacobian of the production rates w.r.t. concentrations and T is accumulated per reaction,
-  with the dense third-body part gathered per distinct efficiency set as rank-1 updates.
+* the Jacobian of the production rates w.r.t. concentrations and T is accumulated per reaction
+  into the caller's matrix (the dense third-body part as a row update per reacting species), so
+  only the concentrations, exp(g/RT) table and rates stay live in registers.
 
 Output: pelec_amd/csrc/gen/mech_<name>.h, struct ``pelec::mech::<name>``.
 """
@@ -141,11 +142,12 @@ class Gen:
                 terms += [f"eg[{k}]"] * int(nu)
             else:
                 terms.append(f"pow(eg[{k}], {f(nu)})")
+        den = []
         for k, nu in sorted(r.reactants.items()):
             if nu == int(nu):
-                terms += [f"ieg[{k}]"] * int(nu)
+                den += [f"eg[{k}]"] * int(nu)
             else:
-                terms.append(f"pow(ieg[{k}], {f(nu)})")
+                den.append(f"pow(eg[{k}], {f(nu)})")
         dnu = sum(r.products.values()) - sum(r.reactants.values())
         if dnu == 1.0:
             terms.append("irefC")
@@ -157,7 +159,8 @@ class Gen:
             terms += ["refC", "refC"]
         elif dnu != 0.0:
             terms.append(f"pow(refC, {f(-dnu)})")
-        return " * ".join(terms) if terms else "1.0"
+        num = " * ".join(terms) if terms else "1.0"
+        return f"({num}) / ({' * '.join(den)})" if den else num
 
     @staticmethod
     def troe_const(troe: List[float]):
@@ -354,11 +357,11 @@ class Gen:
         N, R = self.N, self.R
         w = self.w
         w("  // common T-dependent terms and per-species exp(g/RT) for reverse rates")
-        w("  __host__ __device__ static inline void prelude(double T, double lnT, double invT, double* eg, double* ieg, double* hRT) {")
+        w("  __host__ __device__ static inline void prelude(double T, double lnT, double invT, double* eg, double* hRT) {")
         w(f"    double gRT[{N}];")
         w("    g_h_RT(T, lnT, invT, gRT, hRT);")
         w("#pragma unroll")
-        w(f"    for (int k = 0; k < {N}; ++k) {{ eg[k] = exp(gRT[k]); ieg[k] = 1.0 / eg[k]; }}")
+        w(f"    for (int k = 0; k < {N}; ++k) eg[k] = exp(gRT[k]);")
         w("  }")
         # third-body sums
         def emit_M():
@@ -374,8 +377,8 @@ class Gen:
         w("    const double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
-        w(f"    double eg[{N}], ieg[{N}];")
-        w("    prelude(T, lnT, invT, eg, ieg, nullptr);")
+        w(f"    double eg[{N}];")
+        w("    prelude(T, lnT, invT, eg, nullptr);")
         emit_M()
         for j, r in enumerate(self.m.reactions):
             self.emit_reaction_rate(j, r, False)
@@ -388,8 +391,8 @@ class Gen:
         w("    const double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
-        w(f"    double eg[{N}], ieg[{N}];")
-        w("    prelude(T, lnT, invT, eg, ieg, nullptr);")
+        w(f"    double eg[{N}];")
+        w("    prelude(T, lnT, invT, eg, nullptr);")
         emit_M()
         w("#pragma unroll")
         w(f"    for (int k = 0; k < {N}; ++k) wd[k] = 0.0;")
@@ -407,8 +410,8 @@ class Gen:
         w("    const double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
-        w(f"    double eg[{N}], ieg[{N}], hRT[{N}];")
-        w("    prelude(T, lnT, invT, eg, ieg, hRT);")
+        w(f"    double eg[{N}], hRT[{N}];")
+        w("    prelude(T, lnT, invT, eg, hRT);")
         emit_M()
         w("#pragma unroll")
         w(f"    for (int k = 0; k < {N}; ++k) {{ wd[k] = 0.0; dwdT[k] = 0.0; }}")
@@ -416,12 +419,6 @@ class Gen:
         w(f"    for (int k = 0; k < {N}; ++k)")
         w("#pragma unroll")
         w(f"      for (int m = 0; m < {N}; ++m) dwdC[k * LD + m] = JT(0);")
-        if ne:
-            w(f"    double u[{ne}][{N}];")
-            w("#pragma unroll")
-            w(f"    for (int s = 0; s < {ne}; ++s)")
-            w("#pragma unroll")
-            w(f"      for (int k = 0; k < {N}; ++k) u[s][k] = 0.0;")
         for j, r in enumerate(self.m.reactions):
             self.emit_reaction_rate(j, r, True)
             nu = self.nu_of(r)
@@ -448,27 +445,16 @@ class Gen:
                 for m in sorted(dq):
                     w(f"      dwdC[{k} * LD + {m}] += JT({f(v)} * dq{m});")
             if r.kind in ("three-body", "falloff"):
-                s = self.effsets[tuple(r.efficiencies)]
                 w("      const double dqdM = dkdM * net;")
+                eff = r.efficiencies
                 for k, v in sorted(nu.items()):
-                    w(f"      u[{s}][{k}] += {f(v)} * dqdM;")
+                    for m in range(self.N):
+                        if eff[m] != 0.0:
+                            w(f"      dwdC[{k} * LD + {m}] += JT({f(v * eff[m])} * dqdM);")
+            # one reaction's updates reach memory before the next reaction starts: keeps the
+            # matrix out of registers (it is accumulated in the caller's storage)
+            w("      PELEC_MEMORY_FENCE();")
             w("    }")
-        # rank-1 updates for the efficiency sets
-        for key, s in self.effsets.items():
-            for k in range(N):
-                # only rows that can be nonzero: species appearing in a reaction of this set
-                rows = set()
-                for r in self.m.reactions:
-                    if r.efficiencies is not None and tuple(r.efficiencies) == key:
-                        rows.update(self.nu_of(r).keys())
-                if k not in rows:
-                    continue
-                for m in range(N):
-                    a = key[m]
-                    if a == 0.0:
-                        continue
-                    coef = "" if a == 1.0 else f"{f(a)} * "
-                    w(f"    dwdC[{k} * LD + {m}] += JT({coef}u[{s}][{k}]);")
         w("  }")
 
     # ------------------------------------------------------------------ top level
@@ -482,6 +468,9 @@ class Gen:
         w(f"// species: {N}  reactions: {self.R}")
         w("#pragma once")
         w("#include <cmath>")
+        w("#ifndef PELEC_MEMORY_FENCE")
+        w("#define PELEC_MEMORY_FENCE() __asm__ __volatile__(\"\" ::: \"memory\")")
+        w("#endif")
         w("#ifndef __HIPCC__")
         w("#define __host__")
         w("#define __device__")

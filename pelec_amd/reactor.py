@@ -127,7 +127,7 @@ def array4(t, lo=(0, 0, 0)) -> Array4:
     st = t.stride()
     a.p = t.data_ptr()
     a.nstride, a.kstride, a.jstride = st[0], st[1], st[2]
-    assert st[3] == 1, "x must be the unit-stride dimension"
+    assert st[3] == 1 or nx == 1, "x must be the unit-stride dimension"
     a.begin[:] = lo
     a.end[:] = (lo[0] + nx, lo[1] + ny, lo[2] + nz)
     a.ncomp = nc
