@@ -50,13 +50,30 @@ constexpr int MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10, MAXCOR = 3, MXNCF = 10,
 struct BdfScalars {
   double tn, h, hscale, hprime, eta, etamax, saved_tq5;
   double gamma, gammap, gamrat, crate, acnrm, delp, saved_t;
-  double tau[BDF_QMAX + 2], tq[6], l[BDF_QMAX + 1];
+  double tau[BDF_QMAX + 2], tq[6], l[BDF_QMAX + 2];
   int q, qprime, qwait, L, nstlp, nstlj, indx_acor, jcur;
   int nst, nfe, nje, nsetups, nni, netf, ncfn;
   int m, convfail, nflag, ncf, nef;
 };
 static_assert(sizeof(BdfScalars) % 8 == 0, "BdfScalars must be whole words");
 constexpr int BDF_SCALAR_WORDS = sizeof(BdfScalars) / 8;
+
+// a[idx] for a small array with a run-time index, as selects: every loop over the coefficient
+// arrays below has a static trip count (predicated on q), so the integrator scalars stay in
+// registers instead of private memory.
+template <int K>
+__device__ inline double sel(const double (&a)[K], int idx) {
+  // The empty asm makes every element an opaque register value, so the compiler cannot fold the
+  // select chain back into one load at a computed offset (which would pin the array in scratch).
+  double r = a[0];
+#pragma unroll
+  for (int i = 1; i < K; ++i) {
+    double v = a[i];
+    __asm__("" : "+v"(v));
+    r = (i == idx) ? v : r;
+  }
+  return r;
+}
 
 __device__ inline void scalars_reset(BdfScalars& S, int qmax) {
   S.tn = S.h = S.hscale = S.hprime = 0.0;
@@ -71,7 +88,7 @@ __device__ inline void scalars_reset(BdfScalars& S, int qmax) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) S.tq[i] = 0.0;
 #pragma unroll
-  for (int i = 0; i < BDF_QMAX + 1; ++i) S.l[i] = 0.0;
+  for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
   S.q = 1;
   S.qprime = 1;
   S.L = 2;
@@ -114,9 +131,12 @@ __device__ inline void set_ewt(const V& v, const BdfParams& prm) {
 template <int n, class V>
 __device__ inline void rescale(BdfScalars& S, const V& v) {
   double factor = S.eta;
-  for (int j = 1; j <= S.q; ++j) {
 #pragma unroll
-    for (int i = 0; i < n; ++i) v.Z(j, i) *= factor;
+  for (int j = 1; j <= BDF_QMAX; ++j) {
+    if (j <= S.q) {
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j, i) *= factor;
+    }
     factor *= S.eta;
   }
   S.h = S.hscale * S.eta;
@@ -146,30 +166,37 @@ __device__ inline void set_bdf(BdfScalars& S) {
   double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
   const int q = S.q;
   S.l[0] = S.l[1] = xi_inv = xistar_inv = 1.0;
-  for (int i = 2; i <= q; ++i) S.l[i] = 0.0;
+#pragma unroll
+  for (int i = 2; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
   alpha0 = alpha0_hat = -1.0;
   hsum = S.h;
   if (q > 1) {
-    for (int j = 2; j < q; ++j) {
-      hsum += S.tau[j - 1];
-      xi_inv = S.h / hsum;
-      alpha0 -= 1.0 / j;
-      for (int i = j; i >= 1; --i) S.l[i] += S.l[i - 1] * xi_inv;
+#pragma unroll
+    for (int j = 2; j < BDF_QMAX; ++j) {
+      const bool on = j < q;
+      hsum = on ? hsum + S.tau[j - 1] : hsum;
+      const double xij = S.h / hsum;
+      xi_inv = on ? xij : xi_inv;
+      alpha0 = on ? alpha0 - 1.0 / j : alpha0;
+#pragma unroll
+      for (int i = j; i >= 1; --i) S.l[i] = on ? S.l[i] + S.l[i - 1] * xij : S.l[i];
     }
     alpha0 -= 1.0 / q;
     xistar_inv = -S.l[1] - alpha0;
-    hsum += S.tau[q - 1];
+    hsum += sel(S.tau, q - 1);
     xi_inv = S.h / hsum;
     alpha0_hat = -S.l[1] - xi_inv;
-    for (int i = q; i >= 1; --i) S.l[i] += S.l[i - 1] * xistar_inv;
+#pragma unroll
+    for (int i = BDF_QMAX; i >= 1; --i) S.l[i] = (i <= q) ? S.l[i] + S.l[i - 1] * xistar_inv : S.l[i];
   }
+  const double lq = sel(S.l, q);
   const double A1 = 1.0 - alpha0_hat + alpha0;
   const double A2 = 1.0 + q * A1;
   S.tq[2] = fabs(A1 / (alpha0 * A2));
-  S.tq[5] = fabs(A2 * xistar_inv / (S.l[q] * xi_inv));
+  S.tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
   if (S.qwait == 1) {
     if (q > 1) {
-      const double Cc = xistar_inv / S.l[q];
+      const double Cc = xistar_inv / lq;
       const double A3 = alpha0 + 1.0 / q;
       const double A4 = alpha0_hat + xi_inv;
       const double Cpinv = (1.0 - A4 + A3) / A3;
@@ -177,7 +204,7 @@ __device__ inline void set_bdf(BdfScalars& S) {
     } else {
       S.tq[1] = 1.0;
     }
-    hsum += S.tau[q];
+    hsum += sel(S.tau, q);
     xi_inv = S.h / hsum;
     const double A5 = alpha0 - 1.0 / (q + 1);
     const double A6 = alpha0_hat - xi_inv;
@@ -190,39 +217,55 @@ __device__ inline void set_bdf(BdfScalars& S) {
 template <int n, class V>
 __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax) {
   if (S.q == 2 && dq != 1) return;
+  (void)qmax;
   if (dq == 1) {
-    for (int i = 0; i <= qmax; ++i) S.l[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
     double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = S.hscale;
     S.l[2] = 1.0;
     if (S.q > 1) {
-      for (int j = 1; j < S.q; ++j) {
-        hsum += S.tau[j + 1];
-        const double xi = hsum / S.hscale;
-        prod *= xi;
-        alpha0 -= 1.0 / (j + 1);
-        alpha1 += 1.0 / xi;
-        for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xiold + S.l[i - 1];
-        xiold = xi;
+#pragma unroll
+      for (int j = 1; j < BDF_QMAX; ++j) {
+        const bool on = j < S.q;
+        const double hs = hsum + S.tau[j + 1];
+        const double xi = hs / S.hscale;
+        hsum = on ? hs : hsum;
+        prod = on ? prod * xi : prod;
+        alpha0 = on ? alpha0 - 1.0 / (j + 1) : alpha0;
+        alpha1 = on ? alpha1 + 1.0 / xi : alpha1;
+#pragma unroll
+        for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xiold + S.l[i - 1] : S.l[i];
+        xiold = on ? xi : xiold;
       }
     }
     const double A1 = (-alpha0 - alpha1) / prod;
 #pragma unroll
     for (int i = 0; i < n; ++i) v.Z(S.L, i) = A1 * v.Z(S.indx_acor, i);
-    for (int j = 2; j <= S.q; ++j)
 #pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.Z(S.L, i);
+    for (int j = 2; j <= BDF_QMAX; ++j)
+      if (j <= S.q) {
+#pragma unroll
+        for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.Z(S.L, i);
+      }
   } else {
-    for (int i = 0; i <= qmax; ++i) S.l[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
     S.l[2] = 1.0;
     double hsum = 0.0;
-    for (int j = 1; j <= S.q - 2; ++j) {
-      hsum += S.tau[j];
-      const double xi = hsum / S.hscale;
-      for (int i = j + 2; i >= 2; --i) S.l[i] = S.l[i] * xi + S.l[i - 1];
-    }
-    for (int j = 2; j < S.q; ++j)
 #pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j, i) -= S.l[j] * v.Z(S.q, i);
+    for (int j = 1; j <= BDF_QMAX - 2; ++j) {
+      const bool on = j <= S.q - 2;
+      hsum = on ? hsum + S.tau[j] : hsum;
+      const double xi = hsum / S.hscale;
+#pragma unroll
+      for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xi + S.l[i - 1] : S.l[i];
+    }
+#pragma unroll
+    for (int j = 2; j < BDF_QMAX; ++j)
+      if (j < S.q) {
+#pragma unroll
+        for (int i = 0; i < n; ++i) v.Z(j, i) -= S.l[j] * v.Z(S.q, i);
+      }
   }
 }
 
@@ -335,12 +378,16 @@ __device__ inline void set_eta(BdfScalars& S) {
 template <int n, class V>
 __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qmax) {
   S.nst++;
-  for (int i = S.q; i >= 2; --i) S.tau[i] = S.tau[i - 1];
+#pragma unroll
+  for (int i = BDF_QMAX; i >= 2; --i) S.tau[i] = (i <= S.q) ? S.tau[i - 1] : S.tau[i];
   if (S.q == 1 && S.nst > 1) S.tau[2] = S.tau[1];
   S.tau[1] = S.h;
-  for (int j = 0; j <= S.q; ++j)
 #pragma unroll
-    for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.A(i);
+  for (int j = 0; j <= BDF_QMAX; ++j)
+    if (j <= S.q) {
+#pragma unroll
+      for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.A(i);
+    }
   S.qwait--;
   if (S.qwait == 1 && S.q != qmax) {
 #pragma unroll
