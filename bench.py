@@ -32,6 +32,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_newton launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (profiles/k_newton_<workload>_traffic.txt), or None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"k_newton_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -40,9 +52,10 @@ def host_threads():
     return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
 
 
-def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup):
+def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timing=False):
     r = Reactor(wl.mech)
-    r.configure(rtol=wl.rtol, atol=wl.atol, form=form, lu_fp32=lu_fp32)
+    r.configure(rtol=wl.rtol, atol=wl.atol, form=form, lu_fp32=lu_fp32,
+                kernel_timing=kernel_timing)
     n = d0["T"].shape[0]
     rY, T, rE = d0["rY"].clone(), d0["T"].clone(), d0["rE"].clone()
     FC = torch.zeros_like(T)
@@ -76,6 +89,8 @@ def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup):
     nfail = r.last_failures()
     out = {"rY": rY.cpu().numpy(), "T": T.cpu().numpy(), "diag": diag.cpu().numpy(),
            "times": times, "wall": wall, "nfail": nfail}
+    if kernel_timing:
+        out["kernel_ms"], out["kernel_launches"] = r.kernel_times()
     r.close()
     return out
 
@@ -120,6 +135,10 @@ def main():
                           "cells_per_s": n / float(np.median(o["times"]))}), flush=True)
         return
 
+    # per-kernel device time of the headline form: one extra call with HIP events around every
+    # k_newton / k_jac / k_lu launch (eager launches; the graph replays of the timed calls are
+    # the same kernels on the same streams)
+    kt = time_form(torch, Reactor, wl, d0, "batched", True, 1, 0, kernel_timing=True)
     t_head = float(np.median(head["times"]))
     t_plain = float(np.median(plain["times"]))
     value = n / t_head
@@ -143,10 +162,19 @@ def main():
             "failed_cells": int(o["nfail"]),
         }
 
-    # algorithmic work of the headline kernel (per-cell counters of the last timed call)
+    # algorithmic work of the whole call (per-cell counters of the last timed call)
     dg = head["diag"].astype(np.float64)
     W = flops.work(wl.mech, dg[2].sum(), dg[3].sum(), dg[4].sum(), dg[5].sum(), dg[1].sum())
-    achieved = W / t_head
+    achieved_call = W / t_head
+    # dominant kernel k_newton: each launch does one Newton iteration (RHS + solve) for every
+    # Newton-phase cell, plus the step completion of the cells that converged
+    costs = flops.mechanism_costs(wl.mech)
+    dk = kt["diag"].astype(np.float64)
+    W_newton = (dk[5].sum() * (costs["F_rhs"] + costs["F_solve"]) + dk[1].sum() * costs["F_step"])
+    t_newton = kt["kernel_ms"]["k_newton"] * 1e-3
+    launches = max(1, kt["kernel_launches"])
+    achieved = W_newton / t_newton
+    traffic = measured_traffic(wl.name)
     dgp = plain["diag"].astype(np.float64)
     Wp = flops.work(wl.mech, dgp[2].sum(), dgp[3].sum(), dgp[4].sum(), dgp[5].sum(), dgp[1].sum())
 
@@ -191,8 +219,11 @@ def main():
         "parity_vs_oracle": dict(errs, sample_cells=int(idx.size)),
         "roofline": {"bound": "mfma", "compute": "fp64-valu", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": None, "kernel": "react_queue<float>",
-                     "flops_per_launch": W,
+                     "traffic": traffic, "kernel": "k_newton<float>",
+                     "launches": int(launches), "avg_launch_ms": t_newton / launches * 1e3,
+                     "flops_per_launch": W_newton / launches,
+                     "kernel_ms_total": kt["kernel_ms"],
+                     "whole_call_TFLOPs": achieved_call / 1e12,
                      "cells_steps_mean": float(dg[1].mean()), "cells_steps_max": int(dg[1].max())},
         "cpu_baseline": cpu,
     }

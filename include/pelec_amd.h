@@ -131,6 +131,13 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
  * reactor's previous react call on it (status != 0). */
 int64_t prc_last_failures(prc_reactor* r, void* stream);
 
+/* Per-kernel device time of the previous batched react call, when the reactor was configured
+ * with prc_set_param_int(r, "prc.kernel_timing", 1): ms[0..2] = total milliseconds of the
+ * k_newton, k_jac and k_lu launches (HIP events on the stream each kernel ran on), *launches =
+ * launches of each. Instrumentation with no counterpart in the reference (which only times whole
+ * fabs, Source/React.cpp:168-169). */
+int prc_kernel_times(prc_reactor* r, double* ms, int64_t* launches);
+
 /* ---- EOS helpers at the boundary (Fuego EOS calls PeleC makes around the reactor) -------- */
 /* PYT2RE: rho and e from p, T, Y (Exec/RegTests/zeroD/prob.cpp:34-39). Y component-major. */
 int prc_eos_pyt2re(int64_t ncells, const double* p, const double* T, const double* Y,

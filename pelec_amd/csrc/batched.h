@@ -9,9 +9,11 @@
 //             test, and -- when the correction converged -- the error test, step completion,
 //             order/step choice and the next step's prediction                      (~60 per cell)
 //   k_compact repacks live cells into a dense slot range when half the slots are done.
-// Cells that need a Newton-matrix set-up are appended (wave-aggregated) to a double-buffered list
-// consumed by k_jac/k_lu of the next iteration. The host replays a captured graph of a fixed
-// number of iterations and stops when every started cell is done.
+// Cells that need a Newton-matrix set-up are appended (wave-aggregated) to a rotating list consumed
+// by k_jac/k_lu of the next iteration. Within an iteration the set-up cells and the Newton cells
+// are disjoint, so k_jac -> k_lu and k_newton run concurrently (two branches of the graph). The
+// host replays a captured graph of a fixed number of iterations and stops when every started cell
+// is done.
 //
 // Storage: per-slot structure-of-arrays for vectors and scalars (coalesced: a wave's lanes hold
 // consecutive slots); per-cell array-of-structures for the n x n Jacobian and LU factors, so the
@@ -28,20 +30,29 @@
 
 namespace pelec {
 
-enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2 };
+// PH_READY0 + r: Newton matrix factorised during an iteration i with i % 3 == r. The Newton kernel
+// of iteration i runs concurrently with the set-up kernels of the same iteration, so it takes
+// cells tagged by iteration i - 1 only.
+enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2, PH_READY0 = 3 };
+
+// Set-up lists rotate over three buffers: iteration i consumes list i%3, appends to list (i+1)%3
+// and clears list (i+2)%3 (consumed during i-1, first appended to during i+1).
+constexpr int NLISTS = 3;
 
 // device counters
 enum Ctr : int {
-  CT_SETUP0 = 0,   // length of set-up list 0
-  CT_SETUP1 = 1,   // length of set-up list 1
-  CT_STARTED = 2,  // cells whose integration started
-  CT_DONE = 3,     // started cells finished (ok or failed)
-  CT_FAILED = 4,   // cells with status != 0 (incl. bad input)
-  CT_NACTIVE = 5,  // slots in use [0, nactive)
-  CT_CUR = 6,      // state buffer holding the live slots (0/1)
-  CT_NEWACTIVE = 7,
-  CT_COUNT = 8
+  CT_SETUP0 = 0,   // lengths of set-up lists 0..2
+  CT_STARTED = 3,  // cells whose integration started
+  CT_DONE = 4,     // started cells finished (ok or failed)
+  CT_FAILED = 5,   // cells with status != 0 (incl. bad input)
+  CT_NACTIVE = 6,  // slots in use [0, nactive)
+  CT_CUR = 7,      // state buffer holding the live slots (0/1)
+  CT_NEWACTIVE = 8,
+  CT_COUNT = 9
 };
+
+__host__ __device__ constexpr int list_next(int r) { return r == NLISTS - 1 ? 0 : r + 1; }
+__host__ __device__ constexpr int list_prev(int r) { return r == 0 ? NLISTS - 1 : r - 1; }
 
 struct StateBuf {
   double* zn;       // (QMAX+1) * n * C

@@ -2,15 +2,19 @@
 // Built once per mechanism with -DPELEC_MECH=<ident> -DPELEC_MECH_HEADER="gen/mech_<name>.h".
 //
 // prc.form 0 (plain): one launch of react_plain per box; prc.form 1 (batched): k_start, then a
-// captured graph of ITERS_PER_GRAPH iterations (k_jac, k_lu, k_newton) replayed until every
-// started cell is done, with a compaction of the live slots whenever fewer than half remain
-// alive. prc.lu_fp32 keeps the Newton-matrix factors in FP32 with FP64 refinement.
+// captured graph of ITERS_PER_GRAPH iterations replayed until every started cell is done, with a
+// compaction of the live slots whenever fewer than half remain alive. Each iteration forks into
+// two branches that run concurrently on disjoint cells: k_jac -> k_lu (set-up list) and k_newton
+// (Newton-phase cells). prc.lu_fp32 keeps the Newton-matrix factors in FP32 with FP64 refinement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "pelec_amd.h"
 #include PELEC_MECH_HEADER
@@ -20,7 +24,8 @@ using Mech = pelec::mech::PELEC_MECH;
 #include "reactor_kernels.h"
 
 namespace {
-constexpr int ITERS_PER_GRAPH = 16;  // even: every replay starts on set-up list 0
+constexpr int ITERS_PER_GRAPH = 15;  // a multiple of NLISTS: every replay starts on list 0
+static_assert(ITERS_PER_GRAPH % NLISTS == 0, "replays must start on set-up list 0");
 }
 
 struct prc_reactor {
@@ -50,11 +55,21 @@ struct prc_reactor {
   int ws_src = -1;
   Args ws_layout{};
   hipStream_t cap_stream = nullptr;
+  hipStream_t cap_stream2 = nullptr;  // second branch of each captured iteration
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_lu_fp32 = -1;
   int ncu = 0;
   int occ_newton = 1, occ_jac = 1, occ_lu = 1;
   int64_t last_iterations = 0;
+  // PELEC_AMD_TRACE=1: one stderr line per graph replay of the batched form (live-cell counts)
+  bool trace = false;
+  // prc.kernel_timing: batched iterations launched eagerly with HIP events around every
+  // k_newton / k_jac / k_lu launch (on the stream each runs on); totals of the last call
+  int kernel_timing = 0;
+  double kt_ms[3] = {0.0, 0.0, 0.0};  // newton, jac, lu
+  int64_t kt_launches = 0;
+  std::vector<hipEvent_t> kt_ev;
 };
 
 namespace {
@@ -137,8 +152,8 @@ int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
   const size_t o_J = take(sizeof(double) * NV * NV * cap);
   const size_t o_LU = take(lub * NV * NV * cap);
   const size_t o_perm = take(sizeof(int32_t) * NV * cap);
-  const size_t o_l0 = take(sizeof(int32_t) * cap);
-  const size_t o_l1 = take(sizeof(int32_t) * cap);
+  size_t o_l[NLISTS];
+  for (int b = 0; b < NLISTS; ++b) o_l[b] = take(sizeof(int32_t) * cap);
   const size_t o_remap = take(sizeof(int32_t) * cap);
   void* p = nullptr;
   if (hipMalloc(&p, off) != hipSuccess) return PRC_ENOMEM;
@@ -158,8 +173,7 @@ int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
   L.J = reinterpret_cast<double*>(base + o_J);
   L.LU = base + o_LU;
   L.perm = reinterpret_cast<int32_t*>(base + o_perm);
-  L.list[0] = reinterpret_cast<int32_t*>(base + o_l0);
-  L.list[1] = reinterpret_cast<int32_t*>(base + o_l1);
+  for (int b = 0; b < NLISTS; ++b) L.list[b] = reinterpret_cast<int32_t*>(base + o_l[b]);
   L.remap = reinterpret_cast<int32_t*>(base + o_remap);
   r->ws = p;
   r->ws_bytes = off;
@@ -176,19 +190,29 @@ int capture_graph(prc_reactor* r) {
     (void)hipGraphExecDestroy(r->graph_exec);
     r->graph_exec = nullptr;
   }
-  hipStream_t s = r->cap_stream;
+  hipStream_t s = r->cap_stream, s2 = r->cap_stream2;
   HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const int gj = r->ncu * r->occ_jac;
   const int gl = r->ncu * r->occ_lu;
   const int gn = r->ncu * r->occ_newton;
-  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
-    const int par = it & 1;
+  bool ok = true;
+  for (int it = 0; it < ITERS_PER_GRAPH && ok; ++it) {
+    const int par = it % NLISTS;
+    // fork: Newton branch on s2, set-up branch on s; join before the next iteration
+    ok = hipEventRecord(r->ev_fork, s) == hipSuccess &&
+         hipStreamWaitEvent(s2, r->ev_fork, 0) == hipSuccess;
+    if (!ok) break;
+    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s2, r->d_args, par);
     hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
     hipLaunchKernelGGL(k_lu<LUT>, dim3(gl), dim3(64), 0, s, r->d_args, par);
-    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s, r->d_args, par);
+    ok = hipEventRecord(r->ev_join, s2) == hipSuccess &&
+         hipStreamWaitEvent(s, r->ev_join, 0) == hipSuccess;
   }
   hipGraph_t graph = nullptr;
-  if (hipStreamEndCapture(s, &graph) != hipSuccess) return PRC_EHIP;
+  if (hipStreamEndCapture(s, &graph) != hipSuccess || !ok) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return PRC_EHIP;
+  }
   const hipError_t e = hipGraphInstantiate(&r->graph_exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return PRC_EHIP;
@@ -208,6 +232,57 @@ int occupancy_setup(prc_reactor* r) {
   r->occ_jac = std::max(o, 1);
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_lu<LUT>, 64, 0));
   r->occ_lu = std::max(o, 1);
+  return PRC_OK;
+}
+
+// ITERS_PER_GRAPH iterations launched eagerly with the graph's fork/join structure and timing
+// events around each kernel; the elapsed times are added to r->kt_ms after the replay's sync.
+template <typename LUT>
+int eager_iterations(prc_reactor* r, hipStream_t s) {
+  constexpr int EV = 5;  // newton begin/end on s2; jac begin, jac end = lu begin, lu end on s
+  const size_t need = (size_t)EV * ITERS_PER_GRAPH;
+  while (r->kt_ev.size() < need) {
+    hipEvent_t e = nullptr;
+    HIP_OK(hipEventCreate(&e));
+    r->kt_ev.push_back(e);
+  }
+  hipStream_t s2 = r->cap_stream2;
+  const int gj = r->ncu * r->occ_jac;
+  const int gl = r->ncu * r->occ_lu;
+  const int gn = r->ncu * r->occ_newton;
+  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
+    const int par = it % NLISTS;
+    hipEvent_t* e = r->kt_ev.data() + EV * it;
+    HIP_OK(hipEventRecord(r->ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(s2, r->ev_fork, 0));
+    HIP_OK(hipEventRecord(e[0], s2));
+    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s2, r->d_args, par);
+    HIP_OK(hipEventRecord(e[1], s2));
+    HIP_OK(hipEventRecord(e[2], s));
+    hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
+    HIP_OK(hipEventRecord(e[3], s));
+    hipLaunchKernelGGL(k_lu<LUT>, dim3(gl), dim3(64), 0, s, r->d_args, par);
+    HIP_OK(hipEventRecord(e[4], s));
+    HIP_OK(hipEventRecord(r->ev_join, s2));
+    HIP_OK(hipStreamWaitEvent(s, r->ev_join, 0));
+  }
+  HIP_OK(hipGetLastError());
+  return PRC_OK;
+}
+
+int eager_accumulate(prc_reactor* r) {
+  constexpr int EV = 5;
+  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
+    hipEvent_t* e = r->kt_ev.data() + EV * it;
+    float t = 0.0f;
+    HIP_OK(hipEventElapsedTime(&t, e[0], e[1]));
+    r->kt_ms[0] += t;
+    HIP_OK(hipEventElapsedTime(&t, e[2], e[3]));
+    r->kt_ms[1] += t;
+    HIP_OK(hipEventElapsedTime(&t, e[3], e[4]));
+    r->kt_ms[2] += t;
+  }
+  r->kt_launches += ITERS_PER_GRAPH;
   return PRC_OK;
 }
 
@@ -231,17 +306,32 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
   // safety net: every cell ends within max_steps steps of a bounded number of attempts
   const int64_t max_iters = (int64_t)(r->max_steps + 1) * 64 + 64;
   int64_t nactive = a.ncells;
+  r->kt_ms[0] = r->kt_ms[1] = r->kt_ms[2] = 0.0;
+  r->kt_launches = 0;
   for (;;) {
-    HIP_OK(hipGraphLaunch(r->graph_exec, s));
+    if (r->kernel_timing) {
+      e = eager_iterations<LUT>(r, s);
+      if (e) return e;
+    } else {
+      HIP_OK(hipGraphLaunch(r->graph_exec, s));
+    }
     iters += ITERS_PER_GRAPH;
     HIP_OK(hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    if (r->kernel_timing) {
+      e = eager_accumulate(r);
+      if (e) return e;
+    }
     const int64_t started = (int64_t)r->h_ctr[CT_STARTED];
     const int64_t done = (int64_t)r->h_ctr[CT_DONE];
     if (done >= started) break;
     if (iters > max_iters) return PRC_ESTATE;
     const int64_t alive = started - done;
+    if (r->trace)
+      std::fprintf(stderr, "[pelec_amd] iter %lld started %lld done %lld alive %lld slots %lld\n",
+                   (long long)iters, (long long)started, (long long)done, (long long)alive,
+                   (long long)nactive);
     if (alive * 2 < nactive && nactive > 4096) {
       hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, r->d_args);
       hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, r->d_args, 0);
@@ -289,8 +379,7 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   a.J = L.J;
   a.LU = L.LU;
   a.perm = L.perm;
-  a.list[0] = L.list[0];
-  a.list[1] = L.list[1];
+  for (int b = 0; b < NLISTS; ++b) a.list[b] = L.list[b];
   a.remap = L.remap;
   a.has_src = has_src ? 1 : 0;
   return r->lu_fp32 ? run_batched<float>(r, a, s) : run_batched<double>(r, a, s);
@@ -314,6 +403,9 @@ int launch_1d(K kern, int64_t nc, void* stream, Xs... xs) {
 void release(prc_reactor* r) {
   if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
   if (r->cap_stream) (void)hipStreamDestroy(r->cap_stream);
+  if (r->cap_stream2) (void)hipStreamDestroy(r->cap_stream2);
+  if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
+  if (r->ev_join) (void)hipEventDestroy(r->ev_join);
   if (r->ws) (void)hipFree(r->ws);
   if (r->d_atol) (void)hipFree(r->d_atol);
   if (r->d_ctr) (void)hipFree(r->d_ctr);
@@ -321,7 +413,11 @@ void release(prc_reactor* r) {
   if (r->d_args) (void)hipFree(r->d_args);
   if (r->d_boxes) (void)hipFree(r->d_boxes);
   r->graph_exec = nullptr;
+  for (hipEvent_t e : r->kt_ev) (void)hipEventDestroy(e);
+  r->kt_ev.clear();
   r->cap_stream = nullptr;
+  r->cap_stream2 = nullptr;
+  r->ev_fork = r->ev_join = nullptr;
   r->ws = nullptr;
   r->ws_cap = 0;
   r->ws_lu_fp32 = r->ws_src = -1;
@@ -357,6 +453,8 @@ int prc_create(const char* name, prc_reactor** out) {
     return PRC_EINVAL;
   prc_reactor* r = new (std::nothrow) prc_reactor;
   if (!r) return PRC_ENOMEM;
+  const char* tr = std::getenv("PELEC_AMD_TRACE");
+  r->trace = tr && tr[0] && tr[0] != '0';
   r->name = name;
   r->integrator = kind;
   *out = r;
@@ -374,7 +472,10 @@ int prc_init(prc_reactor* r, int reactor_type, int ncells) {
       release(r);
       return PRC_ENOMEM;
     }
-    if (hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&r->cap_stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess) {
       release(r);
       return PRC_EHIP;
     }
@@ -422,6 +523,8 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
     r->form = v;
   } else if (!std::strcmp(key, "prc.lu_fp32")) {
     r->lu_fp32 = v ? 1 : 0;
+  } else if (!std::strcmp(key, "prc.kernel_timing")) {
+    r->kernel_timing = v ? 1 : 0;
   } else {
     return PRC_EINVAL;
   }
@@ -579,6 +682,13 @@ int64_t prc_last_failures(prc_reactor* r, void* stream) {
     return PRC_EHIP;
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return PRC_EHIP;
   return (int64_t)r->h_ctr[CT_FAILED];
+}
+
+int prc_kernel_times(prc_reactor* r, double* ms, int64_t* launches) {
+  if (!r || !ms) return PRC_EINVAL;
+  for (int k = 0; k < 3; ++k) ms[k] = r->kt_ms[k];
+  if (launches) *launches = r->kt_launches;
+  return PRC_OK;
 }
 
 int prc_eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y, double* rho,

@@ -65,7 +65,7 @@ struct Args {
   double* J;       // per cell n*n (FP64)
   void* LU;        // per cell n*n (LUT)
   int32_t* perm;   // per cell n
-  int32_t* list[2];
+  int32_t* list[NLISTS];
   int32_t* remap;
   int has_src;
 };
@@ -345,11 +345,11 @@ __global__ void __launch_bounds__(64) k_lu(const Args* __restrict__ ap, int par)
           requeue = true;
         }
       } else {
-        st.phase[s] = PH_NEWTON;
+        st.phase[s] = PH_READY0 + par;  // Newton from the next iteration on
       }
       store_scalars(S, st.sc, a.C, s);
     }
-    wave_append(a.list[par ^ 1], &a.ctr[CT_SETUP0 + (par ^ 1)], requeue, (int32_t)s);
+    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], requeue, (int32_t)s);
   }
 }
 
@@ -359,7 +359,8 @@ template <typename LUT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_newton(const Args* __restrict__ ap, int par) {
   __shared__ double gb[NV * 64];
   const Args& a = *ap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + par] = 0;  // consumed by k_jac/k_lu
+  // list (par+2)%3 was consumed by the previous iteration and is first appended to by the next
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
   const int lane = threadIdx.x;
@@ -368,7 +369,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
   constexpr int SH_M = (offsetof(BdfScalars, m) % 8) * 8;
   for (int64_t base = (int64_t)blockIdx.x * 64; base < nact; base += (int64_t)gridDim.x * 64) {
     const int64_t s = base + lane;
-    const bool act = s < nact && st.phase[s] == PH_NEWTON;
+    const int ph = s < nact ? st.phase[s] : PH_DONE;
+    const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
     bool need_setup = false, done = false;
     if (act) {
       const SoAV<NV> v = soa(st, a.C, s);
@@ -475,10 +477,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
                    st.rhoe[a.C + s]);
       } else if (need_setup) {
         st.phase[s] = PH_SETUP;
+      } else if (ph != PH_NEWTON) {
+        st.phase[s] = PH_NEWTON;
       }
       store_scalars(S, st.sc, a.C, s);
     }
-    wave_append(a.list[par ^ 1], &a.ctr[CT_SETUP0 + (par ^ 1)], need_setup, (int32_t)s);
+    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup,
+                (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done);
   }
 }
@@ -527,7 +532,7 @@ __global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap) {
   }
 }
 
-// Renumber the pending set-up list after a compaction.
+// Renumber a pending set-up list after a compaction.
 __global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int par) {
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];

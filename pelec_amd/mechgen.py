@@ -53,8 +53,9 @@ def _ident(name: str) -> str:
 
 
 class Gen:
-    def __init__(self, mech: M.Mechanism):
+    def __init__(self, mech: M.Mechanism, chain: int = 0):
         self.m = mech
+        self.chain = chain
         self.N = mech.nsp
         self.R = mech.nrxn
         self.lines: List[str] = []
@@ -353,6 +354,14 @@ class Gen:
             w("      const double net = Pf;")
         w("      const double qj = kf * net;")
 
+    def emit_chain(self, j: int, dep: str):
+        """After every CHAIN reactions, pass T, lnT and invT through an empty asm that also reads
+        the last rate: the next group's rate constants depend on the asm's outputs, so the
+        compiler cannot hoist all exponentials to the top (which costs one register pair each
+        and spills). Bounds the live set to one group of reactions."""
+        if self.chain > 0 and (j + 1) % self.chain == 0 and j + 1 < self.R:
+            self.w(f"      PELEC_REG_CHAIN(T, lnT, invT, {dep});")
+
     def emit_rates(self):
         N, R = self.N, self.R
         w = self.w
@@ -388,7 +397,7 @@ class Gen:
         # production rates
         w("  // molar production rates wdot_k [mol/(cm^3 s)]")
         w("  __host__ __device__ static inline void wdot(const double* C, double T, double* wd) {")
-        w("    const double lnT = log(T), invT = 1.0 / T;")
+        w("    double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
         w(f"    double eg[{N}];")
@@ -400,6 +409,7 @@ class Gen:
             self.emit_reaction_rate(j, r, False)
             for k, nu in sorted(self.nu_of(r).items()):
                 w(f"      wd[{k}] += {f(nu)} * qj;")
+            self.emit_chain(j, "qj")
             w("    }")
         w("  }")
         # Jacobian
@@ -407,7 +417,7 @@ class Gen:
         w("  // wdot and its Jacobian: dwdC[k*LD+m] = d wdot_k / d C_m, dwdT[k] = d wdot_k / dT")
         w("  template <int LD, typename JT>")
         w("  __host__ __device__ static inline void wdot_jac(const double* C, double T, double* wd, JT* dwdC, double* dwdT) {")
-        w("    const double lnT = log(T), invT = 1.0 / T;")
+        w("    double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
         w(f"    double eg[{N}], hRT[{N}];")
@@ -454,6 +464,7 @@ class Gen:
             # one reaction's updates reach memory before the next reaction starts: keeps the
             # matrix out of registers (it is accumulated in the caller's storage)
             w("      PELEC_MEMORY_FENCE();")
+            self.emit_chain(j, "qj")
             w("    }")
         w("  }")
 
@@ -470,6 +481,13 @@ class Gen:
         w("#include <cmath>")
         w("#ifndef PELEC_MEMORY_FENCE")
         w("#define PELEC_MEMORY_FENCE() __asm__ __volatile__(\"\" ::: \"memory\")")
+        w("#endif")
+        w("#ifndef PELEC_REG_CHAIN")
+        w("#if defined(__HIP_DEVICE_COMPILE__)")
+        w("#define PELEC_REG_CHAIN(a, b, c, d) __asm__ __volatile__(\"\" : \"+v\"(a), \"+v\"(b), \"+v\"(c) : \"v\"(d))")
+        w("#else")
+        w("#define PELEC_REG_CHAIN(a, b, c, d) ((void)(d))")
+        w("#endif")
         w("#endif")
         w("#ifndef __HIPCC__")
         w("#define __host__")

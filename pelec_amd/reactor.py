@@ -74,6 +74,8 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.c_void_p]),
     "prc_last_failures": (_i64, [ctypes.c_void_p, ctypes.c_void_p]),
+    "prc_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(_i64)]),
     "prc_eos_pyt2re": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 6),
     "prc_eos_rey2t": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 5),
     "prc_eos_rty2hi": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 3),
@@ -174,7 +176,7 @@ class Reactor:
             _check(self.lib.prc_set_param_int(self.h, key.encode(), int(value)), f"set {key}")
 
     def configure(self, rtol=None, atol=None, max_order=None, max_steps=None, form=None,
-                  lu_fp32=None):
+                  lu_fp32=None, kernel_timing=None):
         if rtol is not None:
             self.set_param("ode.rtol", float(rtol))
         if atol is not None:
@@ -187,6 +189,8 @@ class Reactor:
             self.set_param("prc.form", {"plain": 0, "batched": 1}.get(form, form))
         if lu_fp32 is not None:
             self.set_param("prc.lu_fp32", 1 if lu_fp32 else 0)
+        if kernel_timing is not None:
+            self.set_param("prc.kernel_timing", 1 if kernel_timing else 0)
         return self
 
     def set_diagnostics(self, diag_tensor):
@@ -250,6 +254,13 @@ class Reactor:
         if n < 0:
             raise ReactorError(f"prc_last_failures: {_ERR.get(n, n)}")
         return n
+
+    def kernel_times(self):
+        """{kernel: total ms} and launches of the previous batched call (kernel_timing mode)."""
+        ms = (ctypes.c_double * 3)()
+        n = _i64(0)
+        _check(self.lib.prc_kernel_times(self.h, ms, ctypes.byref(n)), "prc_kernel_times")
+        return {"k_newton": ms[0], "k_jac": ms[1], "k_lu": ms[2]}, int(n.value)
 
     def close(self):
         if self.h is not None:
