@@ -17,9 +17,8 @@
 //   * the pack/unpack conventions of React.cpp: rho*e(t) = rho*e0 + t * rhoe_src (React.cpp:292),
 //     masked cells (mask == 0) returned untouched with FC = 0.
 //
-// Parity status of each function is stated in DESIGN.md; every function here is pinned by a
-// test in tests/ (golden Cantera trajectories, scipy Radau solutions, invariants, torch autograd
-// and finite differences).
+// Every function here is pinned by a test in tests/ (golden Cantera trajectories, scipy Radau
+// solutions, invariants, torch autograd and finite differences).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>

@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 kernels, called through the C-ABI, against the FP64 CPU oracle.
 
-Tolerances (north_star gate for the reactor outputs): max |dT| <= 1e-3 K and max |dY_k| <= 1e-8
+Tolerances (acceptance gate for the reactor outputs): max |dT| <= 1e-3 K and max |dY_k| <= 1e-8
 against the oracle on identical inputs; EOS helpers to 1e-12 relative; integer/bookkeeping outputs
 (mask, energy update, FC) exact.
 """

@@ -71,3 +71,59 @@ def test_argument_validation_without_gpu():
     assert lib.prc_react_flat(h, None, None, None, None, None, None, None, 4, ctypes.byref(dt),
                               ctypes.byref(tm), None) == -3
     assert lib.prc_destroy(h) == 0
+
+
+SHIM_HARNESS = r"""
+// AMReX-shaped stand-ins: the shim binds to any Box / Array4 with these members.
+#include <cstdio>
+#include "pelec_amd_reactor.H"
+struct Dim3 { int x, y, z; };
+struct Box {
+  int lo[3], hi[3];
+  int smallEnd(int d) const { return lo[d]; }
+  int bigEnd(int d) const { return hi[d]; }
+};
+template <class T> struct Array4 {
+  T* p; long jstride, kstride, nstride; Dim3 begin, end; int ncomp;
+};
+int main() {
+  pelec_amd::Reactor* r = pelec_amd::Reactor::create("ReactorCvode");
+  r->set("ode.rtol", 1e-6);
+  r->set("cvode.max_order", 5);
+  static double buf[4 * 4 * 4 * 32];
+  static int msk[4 * 4 * 4];
+  Box bx{{0, 0, 0}, {3, 3, 3}};
+  Array4<double> a{buf, 4, 16, 64, {0, 0, 0}, {4, 4, 4}, 1};
+  Array4<int> m{msk, 4, 16, 64, {0, 0, 0}, {4, 4, 4}, 1};
+  double dt = 1e-6, time = 0.0;
+  // react before init(1, 1) is a state error (no device work is attempted)
+  const int rc = r->react(bx, a, a, a, a, a, a, m, dt, time);
+  std::printf("%d %d\n", rc, prc_num_species());
+  bool threw = false;
+  try {
+    r->set("no.such.key", 1);
+  } catch (const std::exception&) {
+    threw = true;
+  }
+  delete r;
+  return (rc == PRC_ESTATE && threw) ? 0 : 1;
+}
+"""
+
+
+def test_cxx_shim_compiles_and_forwards(tmp_path):
+    """include/pelec_amd_reactor.H: the ReactorBase-shaped C++ class binds AMReX-shaped types and
+    forwards to the library (host-only calls, no GPU)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "shim.cpp"
+    src.write_text(SHIM_HARNESS)
+    lib = build.lib_path("drm19")
+    exe = tmp_path / "shim"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                    lib, f"-Wl,-rpath,{os.path.dirname(lib)}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.split() == ["-3", "21"]
