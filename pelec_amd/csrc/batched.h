@@ -67,6 +67,7 @@ struct StateBuf {
 
 template <int n>
 struct SoAV {
+  static constexpr bool kMemory = true;
   double* zn;
   double* ac;
   double* ew;

@@ -86,6 +86,7 @@ struct BdfCell {
   int piv[n];
 
   struct Vec {
+    static constexpr bool kMemory = false;
     BdfCell* c;
     __device__ double& Z(int j, int i) const { return c->zn[j][i]; }
     __device__ double& A(int i) const { return c->acor[i]; }

@@ -75,6 +75,29 @@ __device__ inline double sel(const double (&a)[K], int idx) {
   return r;
 }
 
+// Vector accessors declare kMemory = true when Z/A/E live in device memory (the batched form's
+// structure-of-arrays state). The Nordsieck updates below then run one component at a time and
+// end each component with a compiler memory fence: the compiler keeps one component's <= 6
+// history values in registers instead of forwarding the whole (q+1) x n array between passes
+// (which spills). For private-array accessors the fence is omitted.
+template <class V>
+__device__ inline void component_fence() {
+  if constexpr (V::kMemory) PELEC_MEMORY_FENCE();
+}
+
+// Column j of the Nordsieck array for component i, loaded only where j <= q.
+template <class V>
+__device__ inline void load_hist(const V& v, int i, int q, double (&z)[BDF_QMAX + 1]) {
+#pragma unroll
+  for (int j = 0; j <= BDF_QMAX; ++j) z[j] = (j <= q) ? v.Z(j, i) : 0.0;
+}
+template <class V>
+__device__ inline void store_hist(const V& v, int i, int q, const double (&z)[BDF_QMAX + 1]) {
+#pragma unroll
+  for (int j = 0; j <= BDF_QMAX; ++j)
+    if (j <= q) v.Z(j, i) = z[j];
+}
+
 __device__ inline void scalars_reset(BdfScalars& S, int qmax) {
   S.tn = S.h = S.hscale = S.hprime = 0.0;
   S.eta = 1.0;
@@ -130,36 +153,51 @@ __device__ inline void set_ewt(const V& v, const BdfParams& prm) {
 
 template <int n, class V>
 __device__ inline void rescale(BdfScalars& S, const V& v) {
-  double factor = S.eta;
+  double f[BDF_QMAX + 1];
+  f[0] = 1.0;
 #pragma unroll
-  for (int j = 1; j <= BDF_QMAX; ++j) {
-    if (j <= S.q) {
+  for (int j = 1; j <= BDF_QMAX; ++j) f[j] = f[j - 1] * S.eta;
+  const int q = S.q;
 #pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j, i) *= factor;
-    }
-    factor *= S.eta;
+  for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int j = 1; j <= BDF_QMAX; ++j)
+      if (j <= q) v.Z(j, i) *= f[j];
+    component_fence<V>();
   }
   S.h = S.hscale * S.eta;
   S.hscale = S.h;
+}
+
+// Pascal-triangle update of the Nordsieck array (sign +1: predict, -1: undo the prediction),
+// k = 1..q, j = q..k: z_{j-1} += sign z_j -- per component, in registers.
+template <int n, class V>
+__device__ inline void pascal(const V& v, int q, double sign) {
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    double z[BDF_QMAX + 1];
+    load_hist(v, i, q, z);
+#pragma unroll
+    for (int k = 1; k <= BDF_QMAX; ++k)
+#pragma unroll
+      for (int j = BDF_QMAX; j >= k; --j)
+        z[j - 1] = (k <= q && j <= q) ? z[j - 1] + sign * z[j] : z[j - 1];
+    store_hist(v, i, q - 1, z);
+    component_fence<V>();
+  }
 }
 
 template <int n, class V>
 __device__ inline void predict(BdfScalars& S, const V& v, double tstop) {
   S.tn += S.h;
   if ((S.tn - tstop) * S.h > 0.0) S.tn = tstop;
-  for (int k = 1; k <= S.q; ++k)
-    for (int j = S.q; j >= k; --j)
-#pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j - 1, i) += v.Z(j, i);
+  pascal<n>(v, S.q, 1.0);
 }
 
 template <int n, class V>
 __device__ inline void restore(BdfScalars& S, const V& v) {
   S.tn = S.saved_t;
-  for (int k = 1; k <= S.q; ++k)
-    for (int j = S.q; j >= k; --j)
-#pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j - 1, i) -= v.Z(j, i);
+  pascal<n>(v, S.q, -1.0);
 }
 
 __device__ inline void set_bdf(BdfScalars& S) {
@@ -239,14 +277,16 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
       }
     }
     const double A1 = (-alpha0 - alpha1) / prod;
+    const int q = S.q, L = S.L, ia = S.indx_acor;
 #pragma unroll
-    for (int i = 0; i < n; ++i) v.Z(S.L, i) = A1 * v.Z(S.indx_acor, i);
+    for (int i = 0; i < n; ++i) {
+      const double zl = A1 * v.Z(ia, i);
+      v.Z(L, i) = zl;
 #pragma unroll
-    for (int j = 2; j <= BDF_QMAX; ++j)
-      if (j <= S.q) {
-#pragma unroll
-        for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.Z(S.L, i);
-      }
+      for (int j = 2; j <= BDF_QMAX; ++j)
+        if (j <= q) v.Z(j, i) += S.l[j] * zl;
+      component_fence<V>();
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
@@ -260,12 +300,15 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
 #pragma unroll
       for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xi + S.l[i - 1] : S.l[i];
     }
+    const int q = S.q;
 #pragma unroll
-    for (int j = 2; j < BDF_QMAX; ++j)
-      if (j < S.q) {
+    for (int i = 0; i < n; ++i) {
+      const double zq = v.Z(q, i);
 #pragma unroll
-        for (int i = 0; i < n; ++i) v.Z(j, i) -= S.l[j] * v.Z(S.q, i);
-      }
+      for (int j = 2; j < BDF_QMAX; ++j)
+        if (j < q) v.Z(j, i) -= S.l[j] * zq;
+      component_fence<V>();
+    }
   }
 }
 
@@ -382,16 +425,21 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
   for (int i = BDF_QMAX; i >= 2; --i) S.tau[i] = (i <= S.q) ? S.tau[i - 1] : S.tau[i];
   if (S.q == 1 && S.nst > 1) S.tau[2] = S.tau[1];
   S.tau[1] = S.h;
-#pragma unroll
-  for (int j = 0; j <= BDF_QMAX; ++j)
-    if (j <= S.q) {
-#pragma unroll
-      for (int i = 0; i < n; ++i) v.Z(j, i) += S.l[j] * v.A(i);
-    }
   S.qwait--;
-  if (S.qwait == 1 && S.q != qmax) {
+  const bool save_acor = S.qwait == 1 && S.q != qmax;
+  {
+    const int q = S.q;
 #pragma unroll
-    for (int i = 0; i < n; ++i) v.Z(qmax, i) = v.A(i);
+    for (int i = 0; i < n; ++i) {
+      const double ai = v.A(i);
+#pragma unroll
+      for (int j = 0; j <= BDF_QMAX; ++j)
+        if (j <= q) v.Z(j, i) += S.l[j] * ai;
+      if (save_acor) v.Z(qmax, i) = ai;
+      component_fence<V>();
+    }
+  }
+  if (save_acor) {
     S.saved_tq5 = S.tq[5];
     S.indx_acor = qmax;
   }

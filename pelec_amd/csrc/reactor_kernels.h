@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap) {
 }
 
 // Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision)
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_jac(const Args* __restrict__ ap, int par) {
+__global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par) {
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(64) k_lu(const Args* __restrict__ ap, int par)
 // One Newton correction per Newton-phase slot; on convergence the error test, step completion,
 // order/step choice and the next attempt's prediction.
 template <typename LUT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_newton(const Args* __restrict__ ap, int par) {
+__global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int par) {
   __shared__ double gb[NV * 64];
   const Args& a = *ap;
   // list (par+2)%3 was consumed by the previous iteration and is first appended to by the next
