@@ -83,9 +83,12 @@ int prc_init(prc_reactor* r, int reactor_type, int ncells);
 /* ReactorBase::set_typ_vals_ode(typ), Source/React.cpp:11,25: n = NUM_SPECIES + 1 typical values
  * (rhoY_k ..., T); absolute tolerances become atol * typ_i. Pass n = 0 to clear. */
 int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
-/* ParmParse knobs the reactors read (ode.rtol, ode.atol, ode.max_steps, cvode.max_order) and the
- * gfx950 execution form: "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched
- * persistent work-queue kernel; "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
+/* ParmParse knobs the reactors read (ode.rtol, ode.atol, ode.max_steps, cvode.max_order; for
+ * ReactorRK64 rk64.nsubsteps_guess / rk64.nsubsteps_min / rk64.nsubsteps_max, default 10 / 5 /
+ * 100000: first step dt/guess, steps within [dt/max, dt/min]; the pelec.adaptrk_nsubsteps_* knobs
+ * of Exec/Production/ChallengeProblem/challenge.inp:29-31) and the gfx950 execution form:
+ * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
+ * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
 /* Optional per-cell diagnostics (device int32 buffer, PRC_DIAG_NCOMP x capacity, component

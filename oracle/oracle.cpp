@@ -796,6 +796,93 @@ struct Bdf {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Explicit RK64 (the ReactorRK64 drop-in): the six-stage fourth-order two-register low-storage
+// scheme RK4(3)6[2R+]C of Kennedy, Carpenter & Lewis (Appl. Numer. Math. 35 (2000) 177-219), with
+// its embedded third-order error estimate. Register form, per stage s:
+//   k = f(t + c_s h, S);  E += e_s h k;  S = Q + alpha_s h k;  Q = S + beta_s h k
+// starting from Q = S = y_n; y_{n+1} = Q. The controller (the reference's is in the absent
+// PelePhysics submodule) accepts a step when max_i |E_i| / (rtol |y_i| + atol_i) <= 1 or h is at
+// its floor dt / nsub_max, scales h by 0.9 err^(-1/4) within [0.2, 5], keeps dt / nsub_max <= h <=
+// dt / nsub_min, starts from h = dt / nsub_guess and clips the last step to land on dt (within
+// 100 u dt, so accumulated round-off in t never adds a sliver of a step). A step
+// whose stages leave the finite domain is rejected with the factor 0.2; at the floor the cell
+// fails with status 2 (error-test failure).
+constexpr double RK_ALPHA[6] = {3296351145737.0 / 15110423921029.0, 1879360555526.0 / 7321162733569.0,
+                                10797097731880.0 / 20472212111779.0, 754636544611.0 / 15563872110659.0,
+                                3260218886217.0 / 2618290685819.0, 5069185909380.0 / 12292927838509.0};
+constexpr double RK_BETA[6] = {-1204558336989.0 / 10607789004752.0, -3028468927040.0 / 14078136890693.0,
+                               -455570672869.0 / 8930094212428.0, -17275898420483.0 / 15997285579755.0,
+                               -2453906524165.0 / 9868353053862.0, 0.0};
+constexpr double RK_ERR[6] = {-530312978447.0 / 9560368366154.0, 473021958881.0 / 2984707536468.0,
+                              -947229622805.0 / 10456009803779.0, -2921473878215.0 / 13334914072261.0,
+                              1519535112975.0 / 9264196100452.0, 167623581683.0 / 3930932046784.0};
+
+struct Rk64Stats {
+  int nst = 0, nfe = 0, nrej = 0;
+};
+
+int rk64_integrate(CellProblem& P, int n, double* y, double dt, double rtol, const double* atolv,
+                   int nsub_guess, int nsub_min, int nsub_max, int max_steps, Rk64Stats& st) {
+  // stage times c_s (row sums of the Butcher matrix of the register form)
+  double c[6];
+  c[0] = 0.0;
+  for (int s = 1; s < 6; ++s) {
+    double cs = RK_ALPHA[s - 1];
+    for (int j = 0; j + 1 < s; ++j) cs += RK_ALPHA[j] + RK_BETA[j];
+    c[s] = cs;
+  }
+  const double hmin = dt / nsub_max, hmax = dt / nsub_min;
+  double h = dt / nsub_guess;
+  double t = 0.0;
+  std::vector<double> Q(n), S(n), E(n), k(n);
+  const double tend_tol = 100.0 * 2.220446049250313e-16 * dt;
+  while (dt - t > tend_tol) {
+    if (st.nst + st.nrej >= max_steps) return ST_MAXSTEPS;
+    // the last step lands on dt (a remainder below the round-off tolerance is absorbed)
+    const double hu = (t + h >= dt - tend_tol) ? dt - t : h;
+    for (int i = 0; i < n; ++i) {
+      Q[i] = y[i];
+      S[i] = y[i];
+      E[i] = 0.0;
+    }
+    for (int s = 0; s < 6; ++s) {
+      rhs(P, t + c[s] * hu, S.data(), k.data());
+      st.nfe++;
+      for (int i = 0; i < n; ++i) {
+        E[i] += RK_ERR[s] * hu * k[i];
+        S[i] = Q[i] + RK_ALPHA[s] * hu * k[i];
+        Q[i] = S[i] + RK_BETA[s] * hu * k[i];
+      }
+    }
+    double en = 0.0;
+    bool finite = true;
+    for (int i = 0; i < n; ++i) {
+      const double ei = std::fabs(E[i]) / (rtol * std::fabs(y[i]) + atolv[i]);
+      finite = finite && std::isfinite(ei) && std::isfinite(Q[i]);
+      en = std::max(en, ei);
+    }
+    if (!finite) {
+      // a trial stage left the physical domain: shrink, or fail at the step floor
+      if (hu <= hmin) return ST_ERRFAIL;
+      st.nrej++;
+      h = std::max(0.2 * hu, hmin);
+      continue;
+    }
+    const double fac = en > 0.0 ? 0.9 * std::pow(en, -0.25) : 5.0;
+    if (en <= 1.0 || hu <= hmin) {
+      for (int i = 0; i < n; ++i) y[i] = Q[i];
+      t += hu;
+      st.nst++;
+      h = std::min(std::max(hu * std::min(std::max(fac, 0.2), 5.0), hmin), hmax);
+    } else {
+      st.nrej++;
+      h = std::max(hu * std::max(fac, 0.2), hmin);
+    }
+  }
+  return ST_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI (tests only)
@@ -937,6 +1024,69 @@ int orc_react(void* mp, int ncells, double* rY, const double* rYsrc, double* T, 
       o[4] = bdf.st.nni;
       o[5] = bdf.st.netf;
       o[6] = bdf.st.ncfn;
+    }
+  }
+  return nfail;
+}
+
+// ReactorRK64 restatement over ncells cells (same array conventions as orc_react);
+// stats per cell: steps, RHS evaluations, rejected steps.
+int orc_react_rk64(void* mp, int ncells, double* rY, const double* rYsrc, double* T, double* rE,
+                   const double* rEsrc, double* FC, const int* mask, double dt, double rtol,
+                   double atol, const double* typ, int nsub_guess, int nsub_min, int nsub_max,
+                   int max_steps, int nthreads, int* status, int* stats) {
+  const Mech& m = *static_cast<Mech*>(mp);
+  const int N = m.N, n = N + 1;
+  std::vector<double> atolv(n);
+  for (int i = 0; i < n; ++i) atolv[i] = typ ? atol * typ[i] : atol;
+  int nfail = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : nfail)
+#endif
+  for (int c = 0; c < ncells; ++c) {
+    if (mask && mask[c] == 0) {
+      if (FC) FC[c] = 0;
+      if (status) status[c] = 0;
+      continue;
+    }
+    std::vector<double> y(n), src(N);
+    for (int k2 = 0; k2 < N; ++k2) {
+      y[k2] = rY[(size_t)k2 * ncells + c];
+      src[k2] = rYsrc ? rYsrc[(size_t)k2 * ncells + c] : 0.0;
+    }
+    y[N] = T[c];
+    const double rhoe0 = rE[c];
+    const double resrc = rEsrc ? rEsrc[c] : 0.0;
+    CellProblem P{&m, rhoe0, resrc, src.data()};
+    Rk64Stats st;
+    int s = ST_OK;
+    double rho = 0;
+    for (int k2 = 0; k2 < N; ++k2) rho += y[k2];
+    if (!(rho > 0) || !(y[N] > 0) || !(dt > 0)) s = ST_BADINPUT;
+    if (s == ST_OK)
+      s = rk64_integrate(P, n, y.data(), dt, rtol, atolv.data(), nsub_guess, nsub_min, nsub_max,
+                         max_steps, st);
+    if (s == ST_OK) {
+      double rhon = 0;
+      for (int k2 = 0; k2 < N; ++k2) rhon += y[k2];
+      std::vector<double> Y(N);
+      for (int k2 = 0; k2 < N; ++k2) Y[k2] = y[k2] / rhon;
+      const double rEn = rhoe0 + dt * resrc;
+      const double Tn = T_from_e(m, rEn / rhon, Y.data(), y[N], nullptr);
+      for (int k2 = 0; k2 < N; ++k2) rY[(size_t)k2 * ncells + c] = y[k2];
+      T[c] = Tn;
+      rE[c] = rEn;
+    } else {
+      nfail++;
+    }
+    if (FC) FC[c] = st.nfe;
+    if (status) status[c] = s;
+    if (stats) {
+      int* o = stats + (size_t)3 * c;
+      o[0] = st.nst;
+      o[1] = st.nfe;
+      o[2] = st.nrej;
     }
   }
   return nfail;

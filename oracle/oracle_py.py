@@ -52,6 +52,11 @@ def lib():
         L.orc_react.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip]
+        L.orc_react_rk64.restype = ctypes.c_int
+        L.orc_react_rk64.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _ip, _ip]
         _lib = L
     return _lib
 
@@ -173,4 +178,26 @@ class OracleMech:
         nfail = lib().orc_react(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC), _i(msk),
                                 float(dt), float(rtol), float(atol), _d(tv), int(max_steps),
                                 int(max_order), int(nthreads), _i(status), _i(stats))
+        return {"rY": rY, "T": T, "rE": rE, "FC": FC, "status": status, "stats": stats, "nfail": nfail}
+
+    def react_rk64(self, rY, T, rE, dt, rYsrc=None, rEsrc=None, mask=None, rtol=1e-6, atol=1e-10,
+                   typ=None, nsub_guess=10, nsub_min=5, nsub_max=100000, max_steps=100000,
+                   nthreads=0):
+        """ReactorRK64 restatement; stats columns: steps, RHS evaluations, rejected steps."""
+        rY = np.array(rY, dtype=np.float64, order="C", copy=True)
+        T = np.array(T, dtype=np.float64, copy=True)
+        rE = np.array(rE, dtype=np.float64, copy=True)
+        nc = T.shape[0]
+        assert rY.shape == (self.N, nc)
+        src = None if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
+        esrc = None if rEsrc is None else np.ascontiguousarray(rEsrc, dtype=np.float64)
+        msk = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+        tv = None if typ is None else np.ascontiguousarray(typ, dtype=np.float64)
+        FC = np.zeros(nc)
+        status = np.zeros(nc, dtype=np.int32)
+        stats = np.zeros((nc, 3), dtype=np.int32)
+        nfail = lib().orc_react_rk64(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC),
+                                     _i(msk), float(dt), float(rtol), float(atol), _d(tv),
+                                     int(nsub_guess), int(nsub_min), int(nsub_max), int(max_steps),
+                                     int(nthreads), _i(status), _i(stats))
         return {"rY": rY, "T": T, "rE": rE, "FC": FC, "status": status, "stats": stats, "nfail": nfail}

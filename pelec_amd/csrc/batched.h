@@ -156,6 +156,64 @@ __device__ inline int lu_lds(T* a, int* perm) {
   return 0;
 }
 
+// Dense LU with partial pivoting of one n x n matrix held by a group of G lanes (G a power of two,
+// n <= G, groups aligned within the wave), lane r of the group holding row r in registers. Rows
+// are not moved: each lane tracks the position its row would occupy in the row-swapping algorithm
+// (pos), so the pivot of step k is the max |a_ik| over positions >= k with ties to the lower
+// position -- the same choice, and the same factors, as lu_lds. The pivot row is broadcast with
+// cross-lane shuffles. On return lane r holds row pos of the factors (unit-lower multipliers left
+// of the diagonal, U right of it, the diagonal inverted) and perm[pos] = r. Returns 0, or k + 1
+// when column k has no nonzero pivot (every lane of the group returns the same value).
+template <int n, typename T, int G>
+__device__ inline int lu_rows(T (&row)[n], int r, int& pos) {
+  static_assert(n <= G && (G & (G - 1)) == 0 && G <= 64, "one row per lane");
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~(G - 1);
+  pos = r;
+  int ier = 0;
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    // pivot search over positions >= k: key = (|a_rk|, -pos)
+    T v = (r < n && pos >= k) ? T(fabs(row[k])) : T(-1);
+    int kp = pos;
+    int src = lane;
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(v, off, G);
+      const int op = __shfl_xor(kp, off, G);
+      const int os = __shfl_xor(src, off, G);
+      if (ov > v || (ov == v && op < kp)) {
+        v = ov;
+        kp = op;
+        src = os;
+      }
+    }
+    if (!(v > T(0))) {
+      if (ier == 0) ier = k + 1;
+      continue;  // singular: the factors are not used
+    }
+    // the row at position k moves to the pivot row's old position
+    const int pivot_lane = src;
+    const bool is_pivot = lane == pivot_lane;
+    const int old_pos_pivot = kp;
+    if (r < n && pos == k && !is_pivot) pos = old_pos_pivot;
+    if (is_pivot) pos = k;
+    // broadcast the pivot row's columns k..n-1
+    T prow[n];
+#pragma unroll
+    for (int c = k; c < n; ++c) prow[c] = __shfl(row[c], pivot_lane - gbase, G);
+    const T inv = T(1) / prow[k];
+    if (is_pivot) row[k] = inv;
+    if (r < n && pos > k) {
+      const T l = row[k] * inv;
+      row[k] = l;
+#pragma unroll
+      for (int c = k + 1; c < n; ++c) row[c] -= l * prow[c];
+    }
+  }
+  return ier;
+}
+
 // x = (LU)^{-1} P b with factors from lu_lds (AoS block `lu`, inverted diagonal). b in/out (FP64),
 // arithmetic in T. `gb` is this lane's LDS scratch (stride 64) used to gather b by the permutation.
 template <int n, typename T>

@@ -24,13 +24,23 @@ using Mech = pelec::mech::PELEC_MECH;
 #include "reactor_kernels.h"
 
 namespace {
+// Newton-matrix factorisation kernel: one row per lane of a 32-lane group when n <= 32, else one
+// matrix per lane in LDS
+template <typename LUT>
+constexpr auto lu_kernel() {
+  if constexpr (NV <= 32)
+    return &k_lu_rows<LUT>;
+  else
+    return &k_lu<LUT>;
+}
 constexpr int ITERS_PER_GRAPH = 15;  // a multiple of NLISTS: every replay starts on list 0
 static_assert(ITERS_PER_GRAPH % NLISTS == 0, "replays must start on set-up list 0");
 }
 
 struct prc_reactor {
   std::string name;
-  int integrator = 0;  // 0 ReactorCvode (BDF), 2 ReactorNull
+  int integrator = 0;  // 0 ReactorCvode (BDF), 1 ReactorRK64, 2 ReactorNull
+  int rk_guess = 10, rk_min = 5, rk_max = 100000;  // rk64.nsubsteps_{guess,min,max}
   bool initialised = false;
   double rtol = 1e-6, atol = 1e-10;
   int max_steps = 100000;
@@ -204,7 +214,7 @@ int capture_graph(prc_reactor* r) {
     if (!ok) break;
     hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s2, r->d_args, par);
     hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
-    hipLaunchKernelGGL(k_lu<LUT>, dim3(gl), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, r->d_args, par);
     ok = hipEventRecord(r->ev_join, s2) == hipSuccess &&
          hipStreamWaitEvent(s, r->ev_join, 0) == hipSuccess;
   }
@@ -230,7 +240,7 @@ int occupancy_setup(prc_reactor* r) {
   r->occ_newton = std::max(o, 1);
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_jac, 64, 0));
   r->occ_jac = std::max(o, 1);
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_lu<LUT>, 64, 0));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lu_kernel<LUT>(), 64, 0));
   r->occ_lu = std::max(o, 1);
   return PRC_OK;
 }
@@ -261,7 +271,7 @@ int eager_iterations(prc_reactor* r, hipStream_t s) {
     HIP_OK(hipEventRecord(e[2], s));
     hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
     HIP_OK(hipEventRecord(e[3], s));
-    hipLaunchKernelGGL(k_lu<LUT>, dim3(gl), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, r->d_args, par);
     HIP_OK(hipEventRecord(e[4], s));
     HIP_OK(hipEventRecord(r->ev_join, s2));
     HIP_OK(hipStreamWaitEvent(s, r->ev_join, 0));
@@ -350,10 +360,12 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   a.prm.atolv = r->d_atol;
   a.prm.qmax = std::min(std::max(r->max_order, 1), BDF_QMAX);
   a.prm.max_steps = r->max_steps;
+  a.rk = Rk64Params{r->rtol, r->d_atol, r->rk_guess, r->rk_min, r->rk_max, r->max_steps};
+  if (r->integrator == 1 && r->rk_min > r->rk_max) return PRC_EINVAL;
   a.diag = r->diag;
   a.diag_cap = r->diag_cap;
   a.ctr = r->d_ctr;
-  if (a.ncells == 0 || r->form == 0) {
+  if (a.ncells == 0 || r->form == 0 || r->integrator == 1) {
     if (reset_counters) {
       hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, (int64_t)0);
       HIP_OK(hipGetLastError());
@@ -361,7 +373,9 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
     if (a.ncells == 0) return PRC_OK;
     HIP_OK(hipMemcpyAsync(r->d_args, &a, sizeof(Args), hipMemcpyHostToDevice, s));
     const int64_t nb = (a.box.ncells + 63) / 64;
-    if (r->lu_fp32)
+    if (r->integrator == 1)
+      hipLaunchKernelGGL(react_rk64, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+    else if (r->lu_fp32)
       hipLaunchKernelGGL(react_plain<float>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
     else
       hipLaunchKernelGGL(react_plain<double>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
@@ -447,6 +461,8 @@ int prc_create(const char* name, prc_reactor** out) {
   int kind;
   if (!std::strcmp(name, "ReactorCvode"))
     kind = 0;
+  else if (!std::strcmp(name, "ReactorRK64"))
+    kind = 1;
   else if (!std::strcmp(name, "ReactorNull"))
     kind = 2;
   else
@@ -523,6 +539,15 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
     r->form = v;
   } else if (!std::strcmp(key, "prc.lu_fp32")) {
     r->lu_fp32 = v ? 1 : 0;
+  } else if (!std::strcmp(key, "rk64.nsubsteps_guess")) {
+    if (v < 1) return PRC_EINVAL;
+    r->rk_guess = v;
+  } else if (!std::strcmp(key, "rk64.nsubsteps_min")) {
+    if (v < 1) return PRC_EINVAL;
+    r->rk_min = v;
+  } else if (!std::strcmp(key, "rk64.nsubsteps_max")) {
+    if (v < 1) return PRC_EINVAL;
+    r->rk_max = v;
   } else if (!std::strcmp(key, "prc.kernel_timing")) {
     r->kernel_timing = v ? 1 : 0;
   } else {
@@ -615,8 +640,9 @@ int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, doubl
     return PRC_OK;
   }
   hipStream_t s = (hipStream_t)stream;
-  if (r->form == 0) {
-    // plain form: box by box, as the reference loops over MFIter boxes (React.cpp:137-228)
+  if (r->form == 0 || r->integrator == 1) {
+    // plain form (and RK64): box by box, as the reference loops over MFIter boxes
+    // (React.cpp:137-228)
     int64_t first = 0;
     for (int ib = 0; ib < nboxes; ++ib) {
       const prc_box_args& x = boxes[ib];

@@ -12,6 +12,7 @@
 
 #include "batched.h"
 #include "bdf_cell.h"
+#include "rk64.h"
 
 namespace {
 
@@ -56,6 +57,7 @@ struct Args {
   int64_t ncells;       // cells of the call
   double dt;
   BdfParams prm;
+  Rk64Params rk;  // ReactorRK64 knobs (integrator 1)
   int32_t* diag;
   int64_t diag_cap;
   unsigned long long* ctr;  // counters (CT_*)
@@ -94,7 +96,16 @@ __device__ inline const BoxDev& box_of(const Args& a, int64_t g, int64_t& local)
   return a.boxes[lo];
 }
 
-__device__ inline void write_diag(const Args& a, int64_t g, int status, const BdfScalars* S) {
+// Per-cell integration counters reported in the diagnostics buffer (RK64 reports its rejected
+// steps as error-test failures and has no Jacobian, set-up or Newton counts).
+struct Counters {
+  int nst, nfe, nje, nsetups, nni, netf, ncfn;
+};
+__device__ inline Counters counters(const BdfScalars& S) {
+  return Counters{S.nst, S.nfe, S.nje, S.nsetups, S.nni, S.netf, S.ncfn};
+}
+
+__device__ inline void write_diag(const Args& a, int64_t g, int status, const Counters* S) {
   if (!a.diag || g >= a.diag_cap) return;
   int32_t* d = a.diag + g;
   const int64_t s = a.diag_cap;
@@ -111,7 +122,7 @@ __device__ inline void write_diag(const Args& a, int64_t g, int status, const Bd
 // Final write-back of a cell: rhoY, T = T(e_end, Y) with the integrated T as guess, and
 // rho*e = rho*e0 + dt rho*e_src (Source/React.cpp:292 convention). Failed cells keep their inputs.
 template <class ZN>
-__device__ inline void store_cell(const Args& a, int64_t g, int status, const BdfScalars& S,
+__device__ inline void store_cell(const Args& a, int64_t g, int status, const Counters& S,
                                   ZN zn0, double rhoe0, double rhoesrc) {
   int64_t loc;
   const BoxDev& b = box_of(a, g, loc);
@@ -179,7 +190,47 @@ __global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
     s = cell.advance();
   } while (s < 0);
   if (s != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
-  store_cell(a, g, s, cell.S, [&](int e) { return cell.zn[0][e]; }, cell.rhoe0, cell.rhoesrc);
+  store_cell(a, g, s, counters(cell.S), [&](int e) { return cell.zn[0][e]; }, cell.rhoe0, cell.rhoesrc);
+}
+
+// ------------------------------------------------------------------ ReactorRK64 drop-in
+// One cell per lane over a static grid of the box, explicit RK64 with error control (rk64.h).
+__global__ void __launch_bounds__(64) react_rk64(const Args* __restrict__ ap) {
+  const Args& a = *ap;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const BoxDev& b = a.box;
+  if (c >= b.ncells) return;
+  int i, j, k;
+  cell_ijk(b, c, i, j, k);
+  const int64_t g = b.first + c;
+  if (b.mask.p && b.mask.at(i, j, k) == 0) {
+    if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+    write_diag(a, g, CELL_OK, nullptr);
+    return;
+  }
+  Rk64Cell<Mech> cell;
+  double y[NV];
+  double rho = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) {
+    y[s] = b.rY.at(i, j, k, s);
+    rho += y[s];
+  }
+  y[NSP] = b.T.at(i, j, k, 0);
+  cell.rhoe0 = b.rE.at(i, j, k, 0);
+  cell.rhoesrc = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
+  cell.src = b.rYsrc.p ? &b.rYsrc.at(i, j, k, 0) : nullptr;
+  cell.sstride = b.rYsrc.ns;
+  if (!(rho > 0.0) || !(y[NSP] > 0.0) || !(a.dt > 0.0)) {
+    atomicAdd(&a.ctr[CT_FAILED], 1ull);
+    if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
+    write_diag(a, g, CELL_BADINPUT, nullptr);
+    return;
+  }
+  const int st = cell.integrate(y, a.dt, a.rk);
+  if (st != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
+  const Counters cn{cell.nst, cell.nfe, 0, 0, 0, cell.nrej, 0};
+  store_cell(a, g, st, cn, [&](int e) { return y[e]; }, cell.rhoe0, cell.rhoesrc);
 }
 
 // ------------------------------------------------------------------ batched work-queue form
@@ -271,24 +322,34 @@ __global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par
     const int64_t idx = base + threadIdx.x;
     if (idx >= cnt) continue;
     const int64_t s = a.list[par][idx];
-    BdfScalars S;
-    load_scalars(S, st.sc, a.C, s);
-    if (jac_needed(S)) {
-      S.nje++;
-      S.nstlj = S.nst;
-      S.jcur = 1;
+    // bookkeeping first, so no integrator scalar is live across the Jacobian
+    bool need;
+    double tn;
+    {
+      BdfScalars S;
+      load_scalars(S, st.sc, a.C, s);
+      need = jac_needed(S);
+      if (need) {
+        S.nje++;
+        S.nstlj = S.nst;
+        S.jcur = 1;
+      } else {
+        S.jcur = 0;
+      }
+      tn = S.tn;
+      store_scalars(S, st.sc, a.C, s);
+    }
+    PELEC_MEMORY_FENCE();
+    if (need) {
       const int64_t g = st.cell[s];
       const SoAV<NV> v = soa(st, a.C, s);
       double y[NV];
 #pragma unroll
       for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e);
       const double* src = a.has_src ? st.src + s : nullptr;
-      Model::template jac<double>(S.tn, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
+      Model::template jac<double>(tn, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
                                   a.J + g * (int64_t)(NV * NV));
-    } else {
-      S.jcur = 0;
     }
-    store_scalars(S, st.sc, a.C, s);
   }
 }
 
@@ -339,7 +400,78 @@ __global__ void __launch_bounds__(64) k_lu(const Args* __restrict__ ap, int par)
           st.phase[s] = PH_DONE;
           atomicAdd(&a.ctr[CT_FAILED], 1ull);
           atomicAdd(&a.ctr[CT_DONE], 1ull);
-          store_cell(a, g, r, S, [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
+          store_cell(a, g, r, counters(S), [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
+        } else {
+          begin_attempt<NV>(S, v, a.dt);  // nflag = 1: set-up again
+          requeue = true;
+        }
+      } else {
+        st.phase[s] = PH_READY0 + par;  // Newton from the next iteration on
+      }
+      store_scalars(S, st.sc, a.C, s);
+    }
+    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], requeue, (int32_t)s);
+  }
+}
+
+// Newton matrices of the set-up list factorised by 32-lane groups, one matrix per group and one
+// row per lane in registers (lu_rows), two matrices per wave. Used when n <= 32; the per-cell
+// bookkeeping is done by the group's first lane.
+template <typename LUT>
+__global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int par) {
+  constexpr int G = 32;
+  static_assert(NV <= G, "k_lu_rows needs n <= 32");
+  const Args& a = *ap;
+  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  const int lane = threadIdx.x;
+  const int r = lane & (G - 1);
+  const bool leader = r == 0;
+  for (int64_t base = (int64_t)blockIdx.x * 2; base < cnt; base += (int64_t)gridDim.x * 2) {
+    const int64_t idx = base + (lane >> 5);
+    const bool valid = idx < cnt;
+    const int64_t s = valid ? a.list[par][idx] : 0;
+    const int64_t g = valid ? st.cell[s] : 0;
+    constexpr int W_GAMMA = offsetof(BdfScalars, gamma) / 8;
+    double gamma = 0.0;
+    if (valid) {
+      const uint64_t w = st.sc[W_GAMMA * a.C + s];
+      __builtin_memcpy(&gamma, &w, 8);
+    }
+    LUT row[NV];
+    if (valid && r < NV) {
+      const double* Jr = a.J + g * (int64_t)(NV * NV) + r * NV;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        const LUT m = LUT(-gamma * Jr[c]);
+        row[c] = (c == r) ? m + LUT(1) : m;  // I - gamma J
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) row[c] = LUT(0);
+    }
+    int pos = r;
+    int ier = 0;
+    if (valid) ier = lu_rows<NV, LUT, G>(row, r, pos);
+    if (valid && r < NV && ier == 0) {
+      LUT* LUg = reinterpret_cast<LUT*>(a.LU) + g * (int64_t)(NV * NV) + pos * NV;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) LUg[c] = row[c];
+      a.perm[g * NV + pos] = r;
+    }
+    bool requeue = false;
+    if (valid && leader) {
+      BdfScalars S;
+      load_scalars(S, st.sc, a.C, s);
+      const SoAV<NV> v = soa(st, a.C, s);
+      after_setup(S);
+      if (ier != 0) {
+        const int rc = conv_fail<NV>(S, v);
+        if (rc >= 0) {
+          st.phase[s] = PH_DONE;
+          atomicAdd(&a.ctr[CT_FAILED], 1ull);
+          atomicAdd(&a.ctr[CT_DONE], 1ull);
+          store_cell(a, g, rc, counters(S), [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
         } else {
           begin_attempt<NV>(S, v, a.dt);  // nflag = 1: set-up again
           requeue = true;
@@ -473,7 +605,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
         done = true;
         st.phase[s] = PH_DONE;
         if (status != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
-        store_cell(a, g, status, S, [&](int e) { return v.Z(0, e); }, st.rhoe[s],
+        store_cell(a, g, status, counters(S), [&](int e) { return v.Z(0, e); }, st.rhoe[s],
                    st.rhoe[a.C + s]);
       } else if (need_setup) {
         st.phase[s] = PH_SETUP;

@@ -176,7 +176,8 @@ class Reactor:
             _check(self.lib.prc_set_param_int(self.h, key.encode(), int(value)), f"set {key}")
 
     def configure(self, rtol=None, atol=None, max_order=None, max_steps=None, form=None,
-                  lu_fp32=None, kernel_timing=None):
+                  lu_fp32=None, kernel_timing=None, rk64_nsubsteps=None):
+        """rk64_nsubsteps: (guess, min, max) substep counts of ReactorRK64."""
         if rtol is not None:
             self.set_param("ode.rtol", float(rtol))
         if atol is not None:
@@ -191,6 +192,11 @@ class Reactor:
             self.set_param("prc.lu_fp32", 1 if lu_fp32 else 0)
         if kernel_timing is not None:
             self.set_param("prc.kernel_timing", 1 if kernel_timing else 0)
+        if rk64_nsubsteps is not None:
+            g, lo, hi = (int(x) for x in rk64_nsubsteps)
+            self.set_param("rk64.nsubsteps_guess", g)
+            self.set_param("rk64.nsubsteps_min", lo)
+            self.set_param("rk64.nsubsteps_max", hi)
         return self
 
     def set_diagnostics(self, diag_tensor):

@@ -133,6 +133,36 @@ def test_react_batched_vs_oracle(lu_fp32, oracles):
     assert dT <= GATE_T and dY <= GATE_Y
 
 
+@pytest.mark.parametrize("wl,n,dt", [("lidryer_32", 2048, None), ("drm19_64", 512, 1e-6)])
+def test_react_rk64_vs_oracle(wl, n, dt, oracles):
+    """ReactorRK64 drop-in against the oracle's RK64 restatement (same tableau and controller)."""
+    W = inputs.WORKLOADS[wl]
+    dt = W.dt if dt is None else dt
+    d = inputs.make_inputs(wl, n)
+    r = Reactor(W.mech, chem_integrator="ReactorRK64")
+    r.configure(rtol=W.rtol, atol=W.atol)
+    rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+    FC = torch.zeros_like(T)
+    diag = torch.zeros((8, n), dtype=torch.int32, device=DEV)
+    r.set_diagnostics(diag)
+    note(f"react_rk64 {wl} n={n} dt={dt}")
+    r.react_flat(rY, T, rE, dt, FC=FC)
+    torch.cuda.synchronize()
+    assert r.last_failures() == 0
+    r.close()
+    ref = oracles[W.mech].react_rk64(d["rY"], d["T"], d["rE"], dt, rtol=W.rtol, atol=W.atol,
+                                     nsub_guess=10, nsub_min=5, nsub_max=100000, nthreads=8)
+    assert ref["nfail"] == 0
+    out = {"rY": rY.cpu().numpy(), "T": T.cpu().numpy(), "rE": rE.cpu().numpy()}
+    dT, dY, _ = compare(out, ref)
+    note(f"  rk64 max|dT| {dT:.3e} K  max|dY| {dY:.3e}")
+    assert dT <= GATE_T and dY <= GATE_Y
+    assert np.array_equal(out["rE"], ref["rE"])
+    dg = diag.cpu().numpy()
+    assert (dg[0] == 0).all() and (dg[1] > 0).all()
+    assert np.array_equal(FC.cpu().numpy(), dg[2].astype(np.float64))
+
+
 def test_kernel_timing_mode_matches_graph():
     """The eager, event-timed iteration loop runs the same kernels as the graph replay: every
     per-cell result and counter is bit-identical, and the timed kernels report positive time."""

@@ -343,3 +343,82 @@ def test_react_bad_input_reports_status(mechs):
     out = om.react(d["rY"], d["T"], d["rE"], 1e-6, rtol=1e-10, atol=1e-10)
     assert list(out["status"]) == [0, 4, 4, 0]
     assert out["nfail"] == 2
+
+
+# ------------------------------------------------------------------ ReactorRK64 restatement
+# The published RK4(3)6[2R+]C tableau (Kennedy, Carpenter & Lewis 2000) in register form; the
+# order conditions below are the mathematics that fixes it (a transcription slip in any of the 18
+# coefficients breaks at least one of them).
+RK_ALPHA = [3296351145737 / 15110423921029, 1879360555526 / 7321162733569,
+            10797097731880 / 20472212111779, 754636544611 / 15563872110659,
+            3260218886217 / 2618290685819, 5069185909380 / 12292927838509]
+RK_BETA = [-1204558336989 / 10607789004752, -3028468927040 / 14078136890693,
+           -455570672869 / 8930094212428, -17275898420483 / 15997285579755,
+           -2453906524165 / 9868353053862, 0.0]
+RK_ERR = [-530312978447 / 9560368366154, 473021958881 / 2984707536468,
+          -947229622805 / 10456009803779, -2921473878215 / 13334914072261,
+          1519535112975 / 9264196100452, 167623581683 / 3930932046784]
+
+
+def _rk_order_residuals(b, A, c):
+    return np.array([b.sum() - 1, b @ c - 1 / 2, b @ c**2 - 1 / 3, b @ A @ c - 1 / 6,
+                     b @ c**3 - 1 / 4, b @ (c * (A @ c)) - 1 / 8, b @ A @ c**2 - 1 / 12,
+                     b @ A @ A @ c - 1 / 24])
+
+
+def test_rk64_tableau_order_conditions():
+    s = 6
+    A = np.zeros((s, s))
+    for i in range(1, s):
+        for j in range(i):
+            A[i, j] = RK_ALPHA[j] if j == i - 1 else RK_ALPHA[j] + RK_BETA[j]
+    b = np.array(RK_ALPHA) + np.array(RK_BETA)
+    c = A.sum(1)
+    r4 = _rk_order_residuals(b, A, c)
+    assert np.abs(r4).max() < 1e-14, r4                       # fourth order
+    rh = _rk_order_residuals(b - np.array(RK_ERR), A, c)
+    assert np.abs(rh[:4]).max() < 1e-14, rh                   # embedded: third order ...
+    assert np.abs(rh[4:]).max() > 1e-4, rh                    # ... and not fourth
+
+
+def test_rk64_fourth_order_convergence(mechs):
+    """Fixed substeps (guess = min = max): the error against the Radau solution falls by ~2^4
+    per halving of h until it reaches the fixture's own accuracy."""
+    om = mechs["LiDryer"][2]
+    d = np.load(os.path.join(GOLDEN, "radau_lidryer_32.npz"))
+    errs = []
+    for m in (16, 32, 64):
+        r = om.react_rk64(d["rY0"], d["T0"], d["rE0"], float(d["dt"]), rtol=1.0, atol=1.0,
+                          nsub_guess=m, nsub_min=m, nsub_max=m)
+        assert r["nfail"] == 0 and (r["stats"][:, 0] == m).all()
+        errs.append(np.abs(r["T"] - d["T1"]).max())
+    assert errs[0] / errs[1] > 11 and errs[1] / errs[2] > 11, errs
+
+
+@pytest.mark.parametrize("wl,fx,tol_T", [("lidryer_32", "radau_lidryer_32", 1e-6),
+                                         ("drm19_64", "radau_drm19_64", 1e-4)])
+def test_rk64_against_radau(mechs, wl, fx, tol_T):
+    W = inputs.WORKLOADS[wl]
+    d = np.load(os.path.join(GOLDEN, f"{fx}.npz"))
+    r = mechs[W.mech][2].react_rk64(d["rY0"], d["T0"], d["rE0"], float(d["dt"]), rtol=1e-8,
+                                 atol=1e-12, nsub_max=10**6, max_steps=10**6)
+    assert r["nfail"] == 0
+    assert np.abs(r["T"] - d["T1"]).max() < tol_T
+    rho = d["rY1"].sum(0)
+    assert (np.abs(r["rY"] - d["rY1"]) / rho).max() < 1e-8
+    # the energy update is exact and mass is conserved
+    assert np.array_equal(r["rE"], d["rE0"])
+    np.testing.assert_allclose(r["rY"].sum(0), d["rY0"].sum(0), rtol=1e-13)
+
+
+def test_rk64_mask_and_bad_input(mechs):
+    om = mechs["LiDryer"][2]
+    d = inputs.make_inputs("lidryer_32", 8)
+    T = d["T"].copy()
+    T[3] = -5.0
+    mask = np.ones(8, np.int32)
+    mask[5] = 0
+    r = om.react_rk64(d["rY"], T, d["rE"], 1e-7, mask=mask)
+    assert r["status"][3] == 4 and r["nfail"] == 1
+    assert r["status"][5] == 0 and r["FC"][5] == 0
+    assert np.array_equal(r["rY"][:, 5], d["rY"][:, 5]) and r["T"][5] == T[5]
