@@ -130,6 +130,31 @@ typedef struct prc_box_args {
 int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, double* dt_react,
                     double* time, void* stream);
 
+/* ---- PeleC-side passes around the reactor ----------------------------------------------- */
+/* PeleC::react_state for one box (Source/React.cpp:29-362, the per-MFIter body), fused: one pack
+ * kernel (STemp = [rhoY, T, rho e] of S_old -- of S_new when react_init -- and the energy forcing
+ * rhoedot_ext, React.cpp:111-121,181-220), the reactor on the packed box with the species forcing
+ * read in place from nonreact_src (React.cpp:121,222-228), one unpack kernel (new rho, momenta,
+ * rhoY, T, rho e, rho E unless react_init; I_R = [rho wdot_k, rho edot, heat release],
+ * React.cpp:233-314,325-343). Conservative layout of Source/IndexDefines.H: URHO 0, UMX..UMZ 1..3,
+ * UEDEN 4, UEINT 5, UTEMP 6, species from ufs = 7 + NUM_ADV. S_old, S_new, nonreact_src: >= ufs +
+ * NUM_SPECIES comps; I_R: NUM_SPECIES + 2 comps. Optional (p may be NULL): mask (0 = covered cell,
+ * left untouched with I_R = 0, the per-cell form of React.cpp:157-164), rot_energy (0.5 omega^2
+ * r^2 per cell for rotating frames, React.cpp:189-192) and cost (per-cell RHS evaluations of the
+ * reactor: the device-side work estimate that replaces the per-box wall time of
+ * React.cpp:168-169,316-322). On return *time has advanced by *dt_react. */
+int prc_react_state_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3],
+                        prc_array4 S_old, prc_array4 S_new, prc_array4 nonreact_src,
+                        prc_array4 I_R, prc_iarray4 mask, prc_array4 rot_energy, prc_array4 cost,
+                        int ufs, int react_init, double* dt_react, double* time, void* stream);
+
+/* PeleC::computeTemp's per-cell pc_cmpTemp (Source/PeleC.cpp:2254-2272, Source/Utilities.H:68-85):
+ * S(UTEMP) = REY2T(rho, S(UEINT)/rho, Y) over the box, the stored T as the Newton guess; cells
+ * with mask == 0 (covered) are skipped. The internal-energy floor that computeTemp applies first
+ * (reset_internal_energy) is a hydrodynamics fix-up and stays with the caller. */
+int prc_compute_temp(const int32_t lo[3], const int32_t hi[3], prc_array4 S, prc_iarray4 mask,
+                     int ufs, void* stream);
+
 /* Synchronise the stream and return the number of cells whose integration failed in the
  * reactor's previous react call on it (status != 0). */
 int64_t prc_last_failures(prc_reactor* r, void* stream);

@@ -74,6 +74,9 @@ struct prc_reactor {
   int64_t last_iterations = 0;
   // PELEC_AMD_TRACE=1: one stderr line per graph replay of the batched form (live-cell counts)
   bool trace = false;
+  // react_state workspace: rhoY (N), T, rho e, rho e forcing per cell of the box
+  double* sws = nullptr;
+  int64_t sws_cap = 0;
   // prc.kernel_timing: batched iterations launched eagerly with HIP events around every
   // k_newton / k_jac / k_lu launch (on the stream each runs on); totals of the last call
   int kernel_timing = 0;
@@ -441,6 +444,9 @@ void release(prc_reactor* r) {
   r->d_args = nullptr;
   r->d_boxes = nullptr;
   r->d_boxes_cap = 0;
+  if (r->sws) (void)hipFree(r->sws);
+  r->sws = nullptr;
+  r->sws_cap = 0;
   r->graph_lu_fp32 = -1;
   r->ncu = 0;
   r->initialised = false;
@@ -714,6 +720,100 @@ int prc_kernel_times(prc_reactor* r, double* ms, int64_t* launches) {
   if (!r || !ms) return PRC_EINVAL;
   for (int k = 0; k < 3; ++k) ms[k] = r->kt_ms[k];
   if (launches) *launches = r->kt_launches;
+  return PRC_OK;
+}
+
+int prc_react_state_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3],
+                        prc_array4 S_old, prc_array4 S_new, prc_array4 nonreact_src,
+                        prc_array4 I_R, prc_iarray4 mask, prc_array4 rot_energy, prc_array4 cost,
+                        int ufs, int react_init, double* dt_react, double* time, void* stream) {
+  int e = ensure_init(r);
+  if (e) return e;
+  if (!lo || !hi || !dt_react || !time || ufs < UTEMP + 1 || !(*dt_react > 0.0)) return PRC_EINVAL;
+  if (!S_new.p || !I_R.p || !nonreact_src.p || (!react_init && !S_old.p)) return PRC_EINVAL;
+  if ((S_new.ncomp && S_new.ncomp < ufs + NSP) || (I_R.ncomp && I_R.ncomp < NSP + 2))
+    return PRC_EINVAL;
+  const int len0 = hi[0] - lo[0] + 1, len1 = hi[1] - lo[1] + 1, len2 = hi[2] - lo[2] + 1;
+  if (len0 <= 0 || len1 <= 0 || len2 <= 0) {
+    *time += *dt_react;
+    return PRC_OK;
+  }
+  const int64_t nc = (int64_t)len0 * len1 * len2;
+  hipStream_t s = (hipStream_t)stream;
+  if (nc > r->sws_cap) {
+    if (r->sws) (void)hipFree(r->sws);
+    r->sws = nullptr;
+    r->sws_cap = 0;
+    if (hipMalloc(&r->sws, sizeof(double) * (NSP + 3) * nc) != hipSuccess) return PRC_ENOMEM;
+    r->sws_cap = nc;
+  }
+  StateArgs sa{};
+  sa.lo0 = lo[0];
+  sa.lo1 = lo[1];
+  sa.lo2 = lo[2];
+  sa.len0 = len0;
+  sa.len1 = len1;
+  sa.ncells = nc;
+  sa.sold = mk(react_init ? S_new : S_old);
+  sa.snew = mk(S_new);
+  sa.nonrs = mk(nonreact_src);
+  sa.IR = mk(I_R);
+  sa.rot = mk(rot_energy);
+  sa.mask = mki(mask);
+  sa.ufs = ufs;
+  sa.react_init = react_init ? 1 : 0;
+  sa.dt = *dt_react;
+  sa.rY = r->sws;
+  sa.T = r->sws + NSP * nc;
+  sa.rE = r->sws + (NSP + 1) * nc;
+  sa.rEsrc = r->sws + (NSP + 2) * nc;
+  const unsigned nb = (unsigned)((nc + 255) / 256);
+  hipLaunchKernelGGL(k_state_pack, dim3(nb), dim3(256), 0, s, sa);
+  HIP_OK(hipGetLastError());
+  // the reactor integrates the packed box; species sources are read in place from S_nr
+  auto box_view = [&](double* p, int ncomp) {
+    prc_array4 v{};
+    v.p = p;
+    v.jstride = len0;
+    v.kstride = (int64_t)len0 * len1;
+    v.nstride = nc;
+    for (int d = 0; d < 3; ++d) {
+      v.begin[d] = lo[d];
+      v.end[d] = hi[d] + 1;
+    }
+    v.ncomp = ncomp;
+    return v;
+  };
+  prc_array4 src = nonreact_src;
+  src.p = nonreact_src.p + (int64_t)ufs * nonreact_src.nstride;
+  src.ncomp = NSP;
+  Args a{};
+  a.box = make_box(lo, hi, box_view(sa.rY, NSP), src, box_view(sa.T, 1), box_view(sa.rE, 1),
+                   box_view(sa.rEsrc, 1), cost, mask, 0);
+  a.boxes = nullptr;
+  a.nboxes = 1;
+  a.ncells = nc;
+  a.dt = *dt_react;
+  if (r->integrator != 2) {
+    e = launch(r, a, true, s);
+    if (e) return e;
+  }
+  hipLaunchKernelGGL(k_state_unpack, dim3(nb), dim3(256), 0, s, sa);
+  HIP_OK(hipGetLastError());
+  *time += *dt_react;
+  return PRC_OK;
+}
+
+int prc_compute_temp(const int32_t lo[3], const int32_t hi[3], prc_array4 S, prc_iarray4 mask,
+                     int ufs, void* stream) {
+  if (!lo || !hi || !S.p || ufs < UTEMP + 1 || (S.ncomp && S.ncomp < ufs + NSP)) return PRC_EINVAL;
+  const int len0 = hi[0] - lo[0] + 1, len1 = hi[1] - lo[1] + 1, len2 = hi[2] - lo[2] + 1;
+  if (len0 <= 0 || len1 <= 0 || len2 <= 0) return PRC_OK;
+  const int64_t nc = (int64_t)len0 * len1 * len2;
+  hipLaunchKernelGGL(k_cmp_temp, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, lo[0], lo[1], lo[2], len0, len1, nc, mk(S), mki(mask),
+                     ufs);
+  HIP_OK(hipGetLastError());
   return PRC_OK;
 }
 

@@ -688,6 +688,146 @@ __global__ void k_init_counters(unsigned long long* c, int64_t nactive) {
     c[threadIdx.x] = (threadIdx.x == CT_NACTIVE) ? (unsigned long long)nactive : 0ull;
 }
 
+// ------------------------------------------------------------------ react_state pack / unpack
+// PeleC::react_state (Source/React.cpp:29-362) around the reactor, fused into one kernel on each
+// side. Conservative state layout of Source/IndexDefines.H:6-13,60-69: URHO 0, UMX..UMZ 1..3,
+// UEDEN 4, UEINT 5, UTEMP 6, species at ufs = 7 + NUM_ADV.
+constexpr int URHO = 0, UMX = 1, UMY = 2, UMZ = 3, UEDEN = 4, UEINT = 5, UTEMP = 6;
+constexpr double R_ERG = 8.31446261815324e7;
+
+struct StateArgs {
+  int lo0, lo1, lo2, len0, len1;
+  int64_t ncells;
+  A4 sold, snew, nonrs, IR;  // NVAR, NVAR, NVAR, NSP + 2 components
+  A4 rot;                    // optional 0.5 omega^2 r^2 per cell (rotating frame), p may be null
+  A4I mask;                  // optional covered-cell mask (0: skipped), p may be null
+  int ufs;
+  int react_init;
+  double dt;
+  // reactor inputs/outputs on the box (workspace, component-major, ncells per component)
+  double* rY;     // NSP
+  double* T;      // 1
+  double* rE;     // 1
+  double* rEsrc;  // 1
+};
+
+__device__ inline void state_ijk(const StateArgs& a, int64_t c, int& i, int& j, int& k) {
+  i = a.lo0 + (int)(c % a.len0);
+  const int64_t r = c / a.len0;
+  j = a.lo1 + (int)(r % a.len1);
+  k = a.lo2 + (int)(r / a.len1);
+}
+
+// Pack (React.cpp:111-121, 181-220): STemp = [rhoY, T, rho e] of the old state (the new state
+// at initialisation) and the energy forcing rhoedot_ext = (rho E_new - KE_new + rho_new rot -
+// rho_old e_old) / dt with e_old = (rho E_old - KE_old) / rho_old + rot.
+__global__ void k_state_pack(const StateArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  int i, j, k;
+  state_ijk(a, c, i, j, k);
+  if (a.mask.p && a.mask.at(i, j, k) == 0) return;  // covered: not integrated
+  const A4& so = a.react_init ? a.snew : a.sold;
+  const A4& sn = a.snew;
+  const double rot = a.rot.p ? a.rot.at(i, j, k, 0) : 0.0;
+  const double rho_old = so.at(i, j, k, URHO);
+  double mu = so.at(i, j, k, UMX), mv = so.at(i, j, k, UMY), mw = so.at(i, j, k, UMZ);
+  double rinv = 1.0 / rho_old;
+  const double e_old = (so.at(i, j, k, UEDEN) - 0.5 * (mu * mu + mv * mv + mw * mw) * rinv) * rinv + rot;
+  mu = sn.at(i, j, k, UMX);
+  mv = sn.at(i, j, k, UMY);
+  mw = sn.at(i, j, k, UMZ);
+  rinv = 1.0 / sn.at(i, j, k, URHO);
+  const double redot = (sn.at(i, j, k, UEDEN) - 0.5 * (mu * mu + mv * mv + mw * mw) * rinv +
+                        sn.at(i, j, k, URHO) * rot - rho_old * e_old) /
+                       a.dt;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) a.rY[s * a.ncells + c] = so.at(i, j, k, a.ufs + s);
+  a.T[c] = so.at(i, j, k, UTEMP);
+  a.rE[c] = so.at(i, j, k, UEINT);
+  a.rEsrc[c] = redot;
+}
+
+// Unpack (React.cpp:233-314) and heat release (React.cpp:325-343): new rho, momenta, rhoY, T,
+// rho e, rho E (unless initialising), I_R = [(rhoY_new - rhoY_old)/dt - S_nr, d(rho E)/dt - S_nr,
+// -sum h_k(T) I_R,k] with h_k from the updated state.
+__global__ void k_state_unpack(const StateArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  int i, j, k;
+  state_ijk(a, c, i, j, k);
+  if (a.mask.p && a.mask.at(i, j, k) == 0) {
+    // covered: state untouched, I_R = 0 (React.cpp:96-97,157-164)
+#pragma unroll
+    for (int s = 0; s < NSP + 2; ++s) a.IR.at(i, j, k, s) = 0.0;
+    return;
+  }
+  const A4& so = a.react_init ? a.snew : a.sold;
+  const A4& sn = a.snew;
+  const A4& nr = a.nonrs;
+  const double rot = a.rot.p ? a.rot.at(i, j, k, 0) : 0.0;
+  const double rho_old = so.at(i, j, k, URHO);
+  double mu = so.at(i, j, k, UMX), mv = so.at(i, j, k, UMY), mw = so.at(i, j, k, UMZ);
+  const double rinv = 1.0 / rho_old;
+  const double e_old = (so.at(i, j, k, UEDEN) - 0.5 * (mu * mu + mv * mv + mw * mw) * rinv) * rinv + rot;
+  const double redot = a.rEsrc[c];
+  const double um = mu + a.dt * nr.at(i, j, k, UMX);
+  const double vm = mv + a.dt * nr.at(i, j, k, UMY);
+  const double wm = mw + a.dt * nr.at(i, j, k, UMZ);
+  double rhonew = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) rhonew += a.rY[s * a.ncells + c];
+  const double reint = rho_old * e_old + a.dt * redot;
+  const double ke_new = 0.5 * (um * um + vm * vm + wm * wm) / rhonew - rhonew * rot;
+  const double Eold = so.at(i, j, k, UEDEN);
+  double IRk[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+    IRk[s] = (a.rY[s * a.ncells + c] - so.at(i, j, k, a.ufs + s)) / a.dt - nr.at(i, j, k, a.ufs + s);
+  const double IRe = (reint + ke_new - Eold) / a.dt - nr.at(i, j, k, UEDEN);
+  if (!a.react_init) {
+    sn.at(i, j, k, URHO) = rhonew;
+    sn.at(i, j, k, UMX) = um;
+    sn.at(i, j, k, UMY) = vm;
+    sn.at(i, j, k, UMZ) = wm;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) sn.at(i, j, k, a.ufs + s) = a.rY[s * a.ncells + c];
+    sn.at(i, j, k, UTEMP) = a.T[c];
+    sn.at(i, j, k, UEINT) = reint;
+    sn.at(i, j, k, UEDEN) = reint + ke_new;
+  }
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) a.IR.at(i, j, k, s) = IRk[s];
+  a.IR.at(i, j, k, NSP) = IRe;
+  // heat release from the (updated) new state: h_k = e_k + R T / W_k
+  const double Tn = sn.at(i, j, k, UTEMP);
+  double ek[NSP], cvk[NSP];
+  Mech::e_cv(Tn, ek, cvk);
+  double hr = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) hr -= (ek[s] + R_ERG * Tn * Mech::IW[s]) * IRk[s];
+  a.IR.at(i, j, k, NSP + 1) = hr;
+}
+
+// computeTemp's pc_cmpTemp (Source/Utilities.H:68-85, called from Source/PeleC.cpp:2254-2272):
+// T = REY2T(rho, e = rho e / rho, Y) over the full state, the stored T as the Newton guess;
+// covered cells (mask == 0) are skipped.
+__global__ void k_cmp_temp(int lo0, int lo1, int lo2, int len0, int len1, int64_t ncells, A4 S,
+                           A4I mask, int ufs) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int i = lo0 + (int)(c % len0);
+  const int64_t r = c / len0;
+  const int j = lo1 + (int)(r % len1);
+  const int k = lo2 + (int)(r / len1);
+  if (mask.p && mask.at(i, j, k) == 0) return;
+  const double rinv = 1.0 / S.at(i, j, k, URHO);
+  double Y[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) Y[s] = S.at(i, j, k, ufs + s) * rinv;
+  S.at(i, j, k, UTEMP) = Model::T_from_e(S.at(i, j, k, UEINT) * rinv, Y, S.at(i, j, k, UTEMP));
+}
+
 // ------------------------------------------------------------------ EOS helper kernels
 __global__ void eos_pyt2re(int64_t nc, const double* p, const double* T, const double* Y,
                            double* rho, double* e) {

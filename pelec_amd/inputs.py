@@ -141,3 +141,31 @@ def typical_values(rY: np.ndarray, T: np.ndarray, rhoY_min: float = 1e-10) -> np
     typical_rhoY_val_min), typ T = mid-range of T."""
     typ = np.maximum(0.5 * (rY.min(axis=1) + rY.max(axis=1)), rhoY_min)
     return np.concatenate([typ, [0.5 * (T.min() + T.max())]])
+
+
+def make_conservative_state(wl, n, seed=7, hydro=True, ufs=7):
+    """Synthetic PeleC conservative state (NVAR = ufs + N, n) built on make_inputs(wl, n): random
+    velocities (|u| <= 3e3 cm/s), rho E = rho e + KE; and, when hydro, a non-reacting source S_nr
+    perturbing rhoY, momenta and rho E by up to 1e-3 (1e-4 for energy) per dt. Layout of
+    Source/IndexDefines.H: URHO 0, UMX..UMZ 1..3, UEDEN 4, UEINT 5, UTEMP 6, species at ufs."""
+    W = WORKLOADS[wl]
+    d = make_inputs(wl, n)
+    N = d["rY"].shape[0]
+    rng = np.random.default_rng(seed)
+    rho = d["rY"].sum(0)
+    vel = rng.uniform(-3e3, 3e3, (3, n)) if hydro else np.zeros((3, n))
+    S = np.zeros((ufs + N, n))
+    S[0] = rho
+    S[1:4] = rho * vel
+    S[5] = d["rE"]
+    S[4] = d["rE"] + 0.5 * rho * (vel ** 2).sum(0)
+    S[6] = d["T"]
+    S[ufs:ufs + N] = d["rY"]
+    nonrs = np.zeros_like(S)
+    if hydro:
+        dt = W.dt
+        nonrs[ufs:ufs + N] = 1e-3 * d["rY"] * rng.uniform(-1, 1, (N, n)) / dt
+        nonrs[0] = nonrs[ufs:ufs + N].sum(0)
+        nonrs[1:4] = 1e-3 * S[1:4] * rng.uniform(-1, 1, (3, n)) / dt
+        nonrs[4] = 1e-4 * S[4] * rng.uniform(-1, 1, n) / dt
+    return S, nonrs

@@ -76,6 +76,13 @@ _SIGS = {
     "prc_last_failures": (_i64, [ctypes.c_void_p, ctypes.c_void_p]),
     "prc_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(_i64)]),
+    "prc_react_state_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32),
+                                           ctypes.POINTER(_i32), Array4, Array4, Array4, Array4,
+                                           IArray4, Array4, Array4, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "prc_compute_temp": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32), Array4,
+                                        IArray4, ctypes.c_int, ctypes.c_void_p]),
     "prc_eos_pyt2re": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 6),
     "prc_eos_rey2t": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 5),
     "prc_eos_rty2hi": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 3),
@@ -255,6 +262,23 @@ class Reactor:
                                         _stream_handle(stream)), "prc_react_boxes")
         return tm.value
 
+    def react_state_box(self, lo, hi, S_old, S_new, nonreact_src, I_R, dt: float, ufs: int = 7,
+                        react_init: bool = False, mask=None, rot_energy=None, cost=None,
+                        time: float = 0.0, stream=None):
+        """PeleC::react_state on one box. State arrays: torch tensors (NVAR, nz, ny, nx);
+        I_R: (NUM_SPECIES + 2, nz, ny, nx); mask (int32), rot_energy, cost: (nz, ny, nx) or None."""
+        lo_c = (_i32 * 3)(*lo)
+        hi_c = (_i32 * 3)(*hi)
+        dtc = ctypes.c_double(dt)
+        tm = ctypes.c_double(time)
+        _check(self.lib.prc_react_state_box(self.h, lo_c, hi_c, array4(S_old, lo), array4(S_new, lo),
+                                            array4(nonreact_src, lo), array4(I_R, lo),
+                                            iarray4(mask, lo), array4(rot_energy, lo),
+                                            array4(cost, lo), int(ufs), 1 if react_init else 0,
+                                            ctypes.byref(dtc), ctypes.byref(tm),
+                                            _stream_handle(stream)), "prc_react_state_box")
+        return tm.value
+
     def last_failures(self, stream=None) -> int:
         n = self.lib.prc_last_failures(self.h, _stream_handle(stream))
         if n < 0:
@@ -300,6 +324,14 @@ class Eos:
         _check(self.lib.prc_eos_rey2t(T.shape[0], _ptr(rho), _ptr(e), _ptr(Y), _ptr(T),
                                       _stream_handle(stream)), "prc_eos_rey2t")
         return T
+
+    def compute_temp(self, lo, hi, S, ufs: int = 7, mask=None, stream=None):
+        """PeleC::computeTemp's pc_cmpTemp over a box; S (NVAR, nz, ny, nx) updated in place."""
+        lo_c = (_i32 * 3)(*lo)
+        hi_c = (_i32 * 3)(*hi)
+        _check(self.lib.prc_compute_temp(lo_c, hi_c, array4(S, lo), iarray4(mask, lo), int(ufs),
+                                         _stream_handle(stream)), "prc_compute_temp")
+        return S
 
     def rty2hi(self, T, stream=None):
         import torch

@@ -310,3 +310,73 @@ def test_full_size_invariants():
     note(f"full size: steps mean {steps.mean():.1f} max {steps.max()} ignited cells "
          f"{(out['T'] - d['T'] > 100).mean():.3f}")
     assert steps.max() > 20 * max(steps.min(), 1)   # uneven stiffness is really there
+
+
+@pytest.mark.parametrize("form,lu_fp32", [("plain", False), ("batched", True)])
+def test_react_state_box_vs_oracle(form, lu_fp32, oracle_lib):
+    """PeleC::react_state on one 8x8x4 box (S_new = S_old + dt S_nr, covered cells, cost output)
+    against the numpy restatement around the oracle reactor."""
+    from oracle import react_state_np as RS
+    from oracle.kinetics_np import NpMech
+    from pelec_amd.inputs import make_conservative_state as make_state
+    W = inputs.WORKLOADS["drm19_64"]
+    m = M.load("drm19")
+    om, npm = oracle_lib.OracleMech(m), NpMech(m)
+    nx, ny, nz = 8, 8, 4
+    n = nx * ny * nz
+    S, nonrs = make_state("drm19_64", n, seed=11)
+    Snew0 = S + W.dt * nonrs
+    mask = np.ones(n, np.int32)
+    mask[[5, 77, 200]] = 0
+    refS, refIR, refFC = RS.react_state(om, npm, S, Snew0, nonrs, W.dt, mask=mask, rtol=W.rtol,
+                                        atol=W.atol, nthreads=8)
+    shp = lambda a: to_dev(a.reshape(a.shape[0], nz, ny, nx))
+    dS_old, dS_new, dnr = shp(S), shp(Snew0), shp(nonrs)
+    IR = torch.zeros((m.nsp + 2, nz, ny, nx), dtype=torch.float64, device=DEV)
+    cost = torch.zeros((nz, ny, nx), dtype=torch.float64, device=DEV)
+    dmask = to_dev(mask.reshape(nz, ny, nx), torch.int32)
+    r = Reactor("drm19")
+    r.configure(rtol=W.rtol, atol=W.atol, form=form, lu_fp32=lu_fp32)
+    lo, hi = (3, -2, 5), (3 + nx - 1, -2 + ny - 1, 5 + nz - 1)
+    note(f"react_state_box drm19 {nx}x{ny}x{nz} form={form} lu_fp32={lu_fp32}")
+    tnew = r.react_state_box(lo, hi, dS_old, dS_new, dnr, IR, W.dt, mask=dmask, cost=cost)
+    torch.cuda.synchronize()
+    assert abs(tnew - W.dt) < 1e-20 and r.last_failures() == 0
+    r.close()
+    gS = dS_new.cpu().numpy().reshape(S.shape[0], n)
+    gIR = IR.cpu().numpy().reshape(m.nsp + 2, n)
+    N = m.nsp
+    rho = refS[0]
+    assert np.abs(gS[6] - refS[6]).max() <= GATE_T                                 # T
+    assert (np.abs(gS[7:7 + N] - refS[7:7 + N]) / rho).max() <= GATE_Y             # rhoY
+    np.testing.assert_allclose(gS[:6], refS[:6], rtol=1e-12, atol=1e-300)         # rho, rho u, E, e
+    assert (np.abs(gIR[:N] - refIR[:N]) / (rho / W.dt)).max() <= GATE_Y            # rho wdot
+    np.testing.assert_allclose(gIR[N], refIR[N], rtol=1e-9, atol=1e-6 * np.abs(S[4]).max() / W.dt)
+    np.testing.assert_allclose(gIR[N + 1], refIR[N + 1], rtol=1e-4,
+                               atol=1e-6 * np.abs(refIR[N + 1]).max())
+    cov = mask == 0
+    assert np.array_equal(gS[:, cov], Snew0[:, cov]) and (gIR[:, cov] == 0).all()
+    gc = cost.cpu().numpy().reshape(n)
+    assert (gc[cov] == 0).all() and (gc[~cov] > 0).all()
+
+
+def test_compute_temp_vs_oracle():
+    from oracle import react_state_np as RS
+    from oracle.kinetics_np import NpMech
+    from pelec_amd.inputs import make_conservative_state as make_state
+    m = M.load("drm19")
+    npm = NpMech(m)
+    n = 4 * 4 * 4
+    S, _ = make_state("drm19_64", n, seed=5)
+    S[6] *= np.random.default_rng(1).uniform(0.8, 1.2, n)   # stale temperature guess
+    mask = np.ones(n, np.int32)
+    mask[9] = 0
+    ref = RS.compute_temp(npm, S, mask=mask)
+    dS = to_dev(S.reshape(S.shape[0], 4, 4, 4))
+    note("compute_temp drm19 4x4x4")
+    Eos("drm19").compute_temp((0, 0, 0), (3, 3, 3), dS, mask=to_dev(mask.reshape(4, 4, 4), torch.int32))
+    torch.cuda.synchronize()
+    g = dS.cpu().numpy().reshape(S.shape[0], n)
+    np.testing.assert_allclose(g[6], ref[6], rtol=1e-11)
+    assert g[6, 9] == S[6, 9]
+    assert np.array_equal(np.delete(g, 6, 0), np.delete(S, 6, 0))
