@@ -37,7 +37,8 @@ extern "C" {
 /* Per-cell diagnostics record (int32, component-major, PRC_DIAG_NCOMP components):
  * status, steps, RHS evaluations, Jacobian evaluations, LU factorisations, Newton iterations,
  * error-test failures, convergence failures. status: 0 ok, 1 max steps, 2 repeated error-test
- * failures, 3 repeated convergence failures, 4 bad input (rho <= 0, T <= 0 or dt <= 0). */
+ * failures, 3 repeated convergence failures, 4 bad input (rho <= 0, T <= 0 or dt <= 0), 5 adjoint
+ * checkpoint or tape capacity exceeded (prc_adjoint only). */
 #define PRC_DIAG_NCOMP 8
 
 /* Array4-shaped view of caller-owned device memory, mirroring amrex::Array4<Real>:
@@ -154,6 +155,29 @@ int prc_react_state_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3]
  * (reset_internal_energy) is a hydrodynamics fix-up and stays with the caller. */
 int prc_compute_temp(const int32_t lo[3], const int32_t hi[3], prc_array4 S, prc_iarray4 mask,
                      int ufs, void* stream);
+
+/* ---- discrete adjoint (no counterpart in the reference; SUNDIALS CVODES is not built upstream,
+ * CMake/SetSundialsOptions.cmake:5) --------------------------------------------------------- */
+/* Integrates ncells cells (flat, component-major: rY0[k*ncells + c]) over dt with the BDF reactor
+ * (the ReactorCvode algorithm, plain form) at rate multipliers exp(logA[j]) on each reaction's
+ * pre-exponential factor(s) (logA: NUM_REACTIONS device values, NULL = nominal), writes the end
+ * state (rY_out, T_out) and the vector-Jacobian product of the end state with the cotangents
+ * (ct_rY: NUM_SPECIES x ncells, ct_T: ncells; NULL = 0): g_rY0 (d/d rhoY0, NUM_SPECIES x ncells),
+ * g_rE0 (d/d rho*e0, ncells), g_logA (d/d logA_j per cell, NUM_REACTIONS x ncells; the caller
+ * sums over cells). The input T0 is only the Newton guess of T(e, Y): its gradient is zero. The
+ * adjoint is the exact transpose of the discrete integrator with its step-control decisions held
+ * fixed and every corrector equation solved to machine precision (the forward pass polishes each
+ * converged Newton correction with the exact Jacobian, so its end state can differ from
+ * prc_react's by the Newton tolerance), evaluated from checkpoints every "adj.checkpoint_steps" (8) accepted steps with
+ * recomputation ("adj.max_checkpoints" 128 per cell, "adj.segment_records" 256 tape records per
+ * segment; status 5 when exceeded). status: per-cell CellStatus. tape (optional, may be NULL):
+ * the complete forward tape of each cell (tape_cap records of 16 + NUM_SPECIES + 1 doubles per
+ * cell; tape_count[c] = records written, -1 if tape_cap was too small) for verification. */
+int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double* T0,
+                const double* rE0, const double* rYsrc, const double* rEsrc, double dt,
+                const double* logA, const double* ct_rY, const double* ct_T, double* rY_out,
+                double* T_out, double* g_rY0, double* g_rE0, double* g_logA, int32_t* status,
+                double* tape, int64_t tape_cap, int32_t* tape_count, void* stream);
 
 /* Synchronise the stream and return the number of cells whose integration failed in the
  * reactor's previous react call on it (status != 0). */

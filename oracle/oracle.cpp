@@ -50,6 +50,7 @@ struct Mech {
   int N = 0, R = 0;
   std::vector<double> W, Tmid, low, high;
   std::vector<Rxn> rx;
+  std::vector<double> mult;  // optional per-reaction rate multipliers (empty: 1)
 };
 
 // ---------------------------------------------------------------- thermo
@@ -175,6 +176,10 @@ void progress(const Mech& m, const double* C, double T, double* q, double* dqdC,
       const double dlnPr = dlnk0 - dlnkf;
       dlnkdT = dlnkf + dlnPr / (1.0 + Prd) + g * dlnPr + dlnF_dT_fixedPr;
     }
+    if (!m.mult.empty()) {
+      keff *= m.mult[j];
+      dkdM *= m.mult[j];
+    }
     const double net = Pf - Pr * invKc;
     q[j] = keff * net;
     if (dqdC) {
@@ -251,7 +256,7 @@ double rhs(CellProblem& P, double t, const double* y, double* ydot) {
 
 // Exact Jacobian of rhs() (row-major n x n). T is a function of (rhoY, t) through the energy
 // constraint, so the T column is zero and dT/drhoY_m = -e_m / (rho cv) enters every column.
-void jac(CellProblem& P, double t, const double* y, double* J) {
+void jac(CellProblem& P, double t, const double* y, double* J, double* fe = nullptr) {
   const Mech& m = *P.m;
   const int N = m.N, n = N + 1;
   std::vector<double> Y(N), C(N), q(m.R), dqdC((size_t)m.R * N), dqdT(m.R);
@@ -295,6 +300,10 @@ void jac(CellProblem& P, double t, const double* y, double* J) {
   double dNdT = 0;
   for (int k = 0; k < N; ++k) dNdT -= cvk[k] * f[k] + ek[k] * B[k];
   const double dfTdT = (dNdT - fT * dDdT) / D;
+  if (fe) {  // df/d(rho e0) = (df/dT at fixed rhoY) / (rho cv)
+    for (int k = 0; k < N; ++k) fe[k] = B[k] / D;
+    fe[N] = dfTdT / D;
+  }
   for (int mm = 0; mm < N; ++mm) {
     const double tau = -ek[mm] / D;
     double dNdx = 0;
@@ -883,6 +892,180 @@ int rk64_integrate(CellProblem& P, int n, double* y, double dt, double rtol, con
   return ST_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Replay of a recorded BDF step sequence (the tape the device adjoint records; record layout:
+// [0] op [1] q [2] L|save [3] ia|qmax [4] eta|sign|A1|gamma|h [5..10] l_0..l_5 [11] 1/l_1 [12] t
+// [16..16+n) vector). The Nordsieck operations are re-applied with the tape's step-control
+// decisions; each corrector equation D = gamma f(t, z0 + D) - z1 / l_1 is solved to convergence
+// with a fresh Jacobian per Newton iteration, at the given initial state, initial rho*e and rate
+// multipliers exp(logA). This is the map the discrete adjoint differentiates: central differences
+// of it check the adjoint, and its tangent-linear form (one direction) gives forward sensitivities.
+enum { OP_RESCALE = 1, OP_PASCAL, OP_ORDER_UP, OP_ORDER_DOWN, OP_CORRECT, OP_RESTART, OP_INIT };
+
+struct Replay {
+  CellProblem& P;
+  int N, n;
+  bool tangent;
+  std::vector<double> dp;  // tangent direction of logA (R)
+  double de = 0.0;         // tangent direction of rho e0
+  std::vector<double> z, dz, J, fe, M;
+  std::vector<int> piv;
+
+  Replay(CellProblem& P_, bool tan) : P(P_), N(P_.m->N), n(P_.m->N + 1), tangent(tan) {
+    z.assign(6 * n, 0.0);
+    dz.assign(6 * n, 0.0);
+    J.resize(n * n);
+    fe.resize(n);
+    M.resize(n * n);
+    piv.resize(n);
+  }
+  double& Z(int j, int i) { return z[j * n + i]; }
+  double& dZ(int j, int i) { return dz[j * n + i]; }
+
+  // out = J v + (df/dp) dp + fe de at (t, y) (J and fe must be current for (t, y))
+  void tangent_rhs(double t, const double* y, const double* v, double* out) {
+    const Mech& m = *P.m;
+    for (int k = 0; k < n; ++k) {
+      double a = fe[k] * de;
+      for (int j = 0; j < n; ++j) a += J[k * n + j] * v[j];
+      out[k] = a;
+    }
+    std::vector<double> Y(N), C(N), q(m.R), wd(N), ek(N), cvk(N);
+    double rho = 0;
+    for (int k = 0; k < N; ++k) rho += y[k];
+    for (int k = 0; k < N; ++k) {
+      Y[k] = y[k] / rho;
+      C[k] = y[k] / m.W[k];
+    }
+    const double T = T_from_e(m, (P.rhoe0 + t * P.rhoesrc) / rho, Y.data(), y[N], nullptr);
+    progress(m, C.data(), T, q.data(), nullptr, nullptr);
+    for (int j = 0; j < m.R; ++j) q[j] *= dp[j];
+    wdot_from_q(m, q.data(), wd.data());
+    e_cv(m, T, ek.data(), cvk.data());
+    double D = 0, s = 0;
+    for (int k = 0; k < N; ++k) {
+      D += y[k] * cvk[k];
+      const double fk = m.W[k] * wd[k];
+      out[k] += fk;
+      s += ek[k] * fk;
+    }
+    out[N] -= s / D;
+  }
+
+  void op(const double* r) {
+    const int o = (int)r[0], q = (int)r[1];
+    switch (o) {
+      case OP_RESCALE: {
+        double f = 1.0;
+        for (int j = 1; j <= 5; ++j) {
+          f *= r[4];
+          if (j <= q)
+            for (int i = 0; i < n; ++i) {
+              Z(j, i) *= f;
+              dZ(j, i) *= f;
+            }
+        }
+        break;
+      }
+      case OP_PASCAL:
+        for (int k = 1; k <= q; ++k)
+          for (int j = q; j >= k; --j)
+            for (int i = 0; i < n; ++i) {
+              Z(j - 1, i) += r[4] * Z(j, i);
+              dZ(j - 1, i) += r[4] * dZ(j, i);
+            }
+        break;
+      case OP_ORDER_UP: {
+        const int L = (int)r[2], ia = (int)r[3];
+        for (int i = 0; i < n; ++i) {
+          const double zl = r[4] * Z(ia, i), dl = r[4] * dZ(ia, i);
+          Z(L, i) = zl;
+          dZ(L, i) = dl;
+          for (int j = 2; j <= q; ++j) {
+            Z(j, i) += r[5 + j] * zl;
+            dZ(j, i) += r[5 + j] * dl;
+          }
+        }
+        break;
+      }
+      case OP_ORDER_DOWN:
+        for (int i = 0; i < n; ++i) {
+          const double zq = Z(q, i), dq = dZ(q, i);
+          for (int j = 2; j < q; ++j) {
+            Z(j, i) -= r[5 + j] * zq;
+            dZ(j, i) -= r[5 + j] * dq;
+          }
+        }
+        break;
+      case OP_CORRECT: {
+        const bool save = r[2] != 0.0;
+        const int qmax = (int)r[3];
+        const double gamma = r[4], rl1 = r[11], t = r[12];
+        std::vector<double> D(n), y(n), fv(n), g(n);
+        for (int i = 0; i < n; ++i) D[i] = r[16 + i] - Z(0, i);
+        for (int it = 0; it < 40; ++it) {
+          for (int i = 0; i < n; ++i) y[i] = Z(0, i) + D[i];
+          rhs(P, t, y.data(), fv.data());
+          jac(P, t, y.data(), J.data());
+          for (int i = 0; i < n; ++i) g[i] = -(D[i] - gamma * fv[i] + rl1 * Z(1, i));
+          for (int k = 0; k < n * n; ++k) M[k] = -gamma * J[k];
+          for (int k = 0; k < n; ++k) M[k * n + k] += 1.0;
+          getrf(n, M.data(), piv.data());
+          getrs(n, M.data(), piv.data(), g.data());
+          bool conv = true;
+          for (int i = 0; i < n; ++i) {
+            D[i] += g[i];
+            conv = conv && std::fabs(g[i]) <= 1e-14 * std::fabs(Z(0, i) + D[i]) + 1e-300;
+          }
+          if (conv) break;
+        }
+        for (int i = 0; i < n; ++i) y[i] = Z(0, i) + D[i];
+        std::vector<double> dD(n);
+        if (tangent) {
+          jac(P, t, y.data(), J.data(), fe.data());
+          std::vector<double> v(n), tr(n);
+          for (int i = 0; i < n; ++i) v[i] = dZ(0, i);
+          tangent_rhs(t, y.data(), v.data(), tr.data());
+          for (int i = 0; i < n; ++i) dD[i] = gamma * tr[i] - rl1 * dZ(1, i);
+          for (int k = 0; k < n * n; ++k) M[k] = -gamma * J[k];
+          for (int k = 0; k < n; ++k) M[k * n + k] += 1.0;
+          getrf(n, M.data(), piv.data());
+          getrs(n, M.data(), piv.data(), dD.data());
+        }
+        for (int i = 0; i < n; ++i) {
+          for (int j = 0; j <= q; ++j) {
+            Z(j, i) += r[5 + j] * D[i];
+            dZ(j, i) += r[5 + j] * dD[i];
+          }
+          if (save) {
+            Z(qmax, i) = D[i];
+            dZ(qmax, i) = dD[i];
+          }
+        }
+        break;
+      }
+      case OP_RESTART:
+      case OP_INIT: {
+        const double h = r[4], t = r[12];
+        std::vector<double> y(n), fv(n);
+        for (int i = 0; i < n; ++i) y[i] = Z(0, i);
+        rhs(P, t, y.data(), fv.data());
+        if (tangent) {
+          jac(P, t, y.data(), J.data(), fe.data());
+          std::vector<double> v(n), tr(n);
+          for (int i = 0; i < n; ++i) v[i] = dZ(0, i);
+          tangent_rhs(t, y.data(), v.data(), tr.data());
+          for (int i = 0; i < n; ++i) dZ(1, i) = h * tr[i];
+        }
+        for (int i = 0; i < n; ++i) Z(1, i) = h * fv[i];
+        break;
+      }
+      default:
+        break;
+    }
+  }
+};
+
 }  // namespace
 
 // ====================================================================== C ABI (tests only)
@@ -1090,6 +1273,54 @@ int orc_react_rk64(void* mp, int ncells, double* rY, const double* rYsrc, double
     }
   }
   return nfail;
+}
+
+// Replay one cell's tape (nrec records of rs doubles) from y0 (n = N + 1: rhoY, T guess), rho e0
+// and logA (R values, null = 0); writes y_end (n) and T_end. With dy0 non-null also the tangent:
+// dy_end / dT_end along (dy0, drhoe0, dlogA).
+int orc_replay(void* mp, const double* tape, int nrec, int rs, const double* y0, double rhoe0,
+               double rhoesrc, const double* rYsrc, const double* logA, double dt, double* yend,
+               double* Tend, const double* dy0, double drhoe0, const double* dlogA, double* dyend,
+               double* dTend) {
+  Mech m = *static_cast<Mech*>(mp);
+  const int N = m.N, n = N + 1;
+  m.mult.assign(m.R, 1.0);
+  if (logA)
+    for (int j = 0; j < m.R; ++j) m.mult[j] = std::exp(logA[j]);
+  std::vector<double> src(N, 0.0);
+  if (rYsrc)
+    for (int k = 0; k < N; ++k) src[k] = rYsrc[k];
+  CellProblem P{&m, rhoe0, rhoesrc, src.data()};
+  Replay rp(P, dy0 != nullptr);
+  rp.dp.assign(m.R, 0.0);
+  if (dlogA)
+    for (int j = 0; j < m.R; ++j) rp.dp[j] = dlogA[j];
+  rp.de = drhoe0;
+  for (int r = 0; r < nrec; ++r) {
+    const double* rec = tape + (size_t)r * rs;
+    if ((int)rec[0] == OP_INIT)
+      for (int i = 0; i < n; ++i) {
+        rp.Z(0, i) = y0[i];
+        rp.dZ(0, i) = dy0 ? dy0[i] : 0.0;
+      }
+    rp.op(rec);
+  }
+  for (int i = 0; i < n; ++i) yend[i] = rp.Z(0, i);
+  double rho = 0;
+  for (int k = 0; k < N; ++k) rho += yend[k];
+  std::vector<double> Y(N), ek(N), cvk(N);
+  for (int k = 0; k < N; ++k) Y[k] = yend[k] / rho;
+  const double T = T_from_e(m, (rhoe0 + dt * rhoesrc) / rho, Y.data(), yend[N], nullptr);
+  *Tend = T;
+  if (dy0) {
+    e_cv(m, T, ek.data(), cvk.data());
+    double D = 0, dT = drhoe0;
+    for (int k = 0; k < N; ++k) D += yend[k] * cvk[k];
+    for (int k = 0; k < N; ++k) dT -= ek[k] * rp.dZ(0, k);
+    for (int i = 0; i < n; ++i) dyend[i] = rp.dZ(0, i);
+    *dTend = dT / D;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -57,6 +57,10 @@ def lib():
                                      _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _ip, _ip]
+        L.orc_replay.restype = ctypes.c_int
+        L.orc_replay.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                 ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_double,
+                                 _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -179,6 +183,32 @@ class OracleMech:
                                 float(dt), float(rtol), float(atol), _d(tv), int(max_steps),
                                 int(max_order), int(nthreads), _i(status), _i(stats))
         return {"rY": rY, "T": T, "rE": rE, "FC": FC, "status": status, "stats": stats, "nfail": nfail}
+
+    def replay(self, tape, y0, rhoe0, dt, logA=None, rhoesrc=0.0, rYsrc=None, tangent=None):
+        """Replay one cell's recorded BDF tape (nrec, 16 + n) at (y0, rhoe0, logA). Returns
+        (y_end, T_end); with tangent = (dy0, drhoe0, dlogA) also (dy_end, dT_end)."""
+        tape = np.ascontiguousarray(tape, dtype=np.float64)
+        n = self.N + 1
+        assert tape.ndim == 2 and tape.shape[1] == 16 + n
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        la = None if logA is None else np.ascontiguousarray(logA, dtype=np.float64)
+        src = None if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
+        yend = np.zeros(n)
+        Tend = np.zeros(1)
+        dy0 = de = dla = None
+        dyend = dTend = None
+        if tangent is not None:
+            dy0 = np.ascontiguousarray(tangent[0], dtype=np.float64)
+            de = float(tangent[1])
+            dla = None if tangent[2] is None else np.ascontiguousarray(tangent[2], dtype=np.float64)
+            dyend = np.zeros(n)
+            dTend = np.zeros(1)
+        lib().orc_replay(self.h, _d(tape), tape.shape[0], tape.shape[1], _d(y0), float(rhoe0),
+                         float(rhoesrc), _d(src), _d(la), float(dt), _d(yend), _d(Tend), _d(dy0),
+                         de if de is not None else 0.0, _d(dla), _d(dyend), _d(dTend))
+        if tangent is None:
+            return yend, Tend[0]
+        return yend, Tend[0], dyend, dTend[0]
 
     def react_rk64(self, rY, T, rE, dt, rYsrc=None, rEsrc=None, mask=None, rtol=1e-6, atol=1e-10,
                    typ=None, nsub_guess=10, nsub_min=5, nsub_max=100000, max_steps=100000,

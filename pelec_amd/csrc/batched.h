@@ -66,7 +66,7 @@ struct StateBuf {
 };
 
 template <int n>
-struct SoAV {
+struct SoAV : NoTape {
   static constexpr bool kMemory = true;
   double* zn;
   double* ac;

@@ -65,7 +65,23 @@ __device__ inline void lu_solve(const T* a, const int* piv, double* b) {
   for (int i = 0; i < n; ++i) b[i] = double(x[i]);
 }
 
-template <class Mech, typename LUT>
+// Tape (default NoTapeRec: nothing recorded) receives the Nordsieck operations of the step
+// sequence for the discrete adjoint; MUL scales reaction j's rate constant by rmul[j].
+struct NoTapeRec {
+  template <class V> __device__ void rec_rescale(const V&, int, double) {}
+  template <class V> __device__ void rec_pascal(const V&, int, double) {}
+  template <class V> __device__ void rec_order_up(const V&, int, int, int, double, const double*) {}
+  template <class V> __device__ void rec_order_down(const V&, int, const double*) {}
+  template <class V> __device__ void rec_correct(const V&, const BdfScalars&, bool, int) {}
+  template <class V> __device__ void rec_restart(const V&, double, double) {}
+  template <class V> __device__ void rec_init(const V&, double) {}
+  __device__ void rec_mark_save() {}
+};
+
+// POLISH: after the modified Newton iteration converges, the corrector equation is solved to
+// machine precision with the exact Jacobian (the discrete adjoint linearises the exactly solved
+// step map, so its forward trajectory must be that map's).
+template <class Mech, typename LUT, class Tape = NoTapeRec, bool MUL = false, bool POLISH = false>
 struct BdfCell {
   using Model = CellModel<Mech>;
   static constexpr int N = Mech::NSP;
@@ -84,6 +100,10 @@ struct BdfCell {
   double savedJ[n * n];
   LUT M[n * n];
   int piv[n];
+  Tape tape;
+  const double* rmul = nullptr;  // per-reaction rate multipliers (MUL)
+  double Jx[POLISH ? n * n : 1];  // exact Jacobian / Newton matrix of the polish
+  int pivx[POLISH ? n : 1];
 
   struct Vec {
     static constexpr bool kMemory = false;
@@ -91,12 +111,26 @@ struct BdfCell {
     __device__ double& Z(int j, int i) const { return c->zn[j][i]; }
     __device__ double& A(int i) const { return c->acor[i]; }
     __device__ double& E(int i) const { return c->ewt[i]; }
+    __device__ void rec_rescale(int q, double eta) const { c->tape.rec_rescale(*this, q, eta); }
+    __device__ void rec_pascal(int q, double sign) const { c->tape.rec_pascal(*this, q, sign); }
+    __device__ void rec_order_up(int q, int L, int ia, double A1, const double* l) const {
+      c->tape.rec_order_up(*this, q, L, ia, A1, l);
+    }
+    __device__ void rec_order_down(int q, const double* l) const {
+      c->tape.rec_order_down(*this, q, l);
+    }
+    __device__ void rec_correct(const BdfScalars& S, bool save, int qmax) const {
+      c->tape.rec_correct(*this, S, save, qmax);
+    }
+    __device__ void rec_restart(double h, double t) const { c->tape.rec_restart(*this, h, t); }
+    __device__ void rec_init(double h) const { c->tape.rec_init(*this, h); }
+    __device__ void rec_mark_save() const { c->tape.rec_mark_save(); }
   };
   __device__ Vec vec() { return Vec{this}; }
 
   __device__ void f(double t, const double* yy, double* out) {
     S.nfe++;
-    Model::rhs(t, yy, rhoe0, rhoesrc, src, sstride, out);
+    Model::template rhs<MUL>(t, yy, rhoe0, rhoesrc, src, sstride, out, rmul);
   }
 
   __device__ int lsetup() {
@@ -104,7 +138,8 @@ struct BdfCell {
       S.nje++;
       S.nstlj = S.nst;
       S.jcur = 1;
-      Model::template jac<double>(S.tn, zn[0], rhoe0, rhoesrc, src, sstride, savedJ);
+      Model::template jac<double, MUL>(S.tn, zn[0], rhoe0, rhoesrc, src, sstride, savedJ, nullptr,
+                                       rmul);
     } else {
       S.jcur = 0;
     }
@@ -157,11 +192,38 @@ struct BdfCell {
         for (int i = 0; i < n; ++i) y[i] = zn[0][i] + acor[i];
         f(S.tn, y, ftemp);
       }
-      if (r == 0) return 0;
+      if (r == 0) {
+        if constexpr (POLISH) polish();
+        return 0;
+      }
       if (r == 3) return 3;
       callSetup = true;  // stale Jacobian: rebuild and retry
       S.convfail = 2;
     }
+  }
+
+  // acor = gamma f(tn, zn0 + acor) - zn1 / l1 solved to convergence (exact Jacobian each iteration)
+  __device__ void polish() {
+    const double rl1 = 1.0 / S.l[1];
+    for (int it = 0; it < 8; ++it) {
+      double y[n], g[n];
+#pragma unroll
+      for (int i = 0; i < n; ++i) y[i] = zn[0][i] + acor[i];
+      f(S.tn, y, g);
+      for (int i = 0; i < n; ++i) g[i] = S.gamma * g[i] - rl1 * zn[1][i] - acor[i];
+      Model::template jac<double, MUL>(S.tn, y, rhoe0, rhoesrc, src, sstride, Jx, nullptr, rmul);
+      for (int k = 0; k < n * n; ++k) Jx[k] = -S.gamma * Jx[k];
+      for (int k = 0; k < n; ++k) Jx[k * n + k] += 1.0;
+      if (lu_factor<n, double>(Jx, pivx) != 0) break;
+      lu_solve<n, double>(Jx, pivx, g);
+      bool conv = true;
+      for (int i = 0; i < n; ++i) {
+        acor[i] += g[i];
+        conv = conv && fabs(g[i]) <= 1e-14 * fabs(zn[0][i] + acor[i]) + 1e-300;
+      }
+      if (conv) break;
+    }
+    S.acnrm = wrms_acor<n>(vec());
   }
 
   // cvStep
@@ -182,6 +244,7 @@ struct BdfCell {
       const int s = error_test_fail<n>(S, vec(), dsm, prm.qmax);
       if (s >= 0) return s;
       if (s == -2) {
+        vec().rec_restart(S.h, S.tn);
         double t[n];
         f(S.tn, zn[0], t);
 #pragma unroll
@@ -203,6 +266,7 @@ struct BdfCell {
     S.hprime = S.h;
 #pragma unroll
     for (int i = 0; i < n; ++i) zn[1][i] *= S.h;
+    vec().rec_init(S.h);
   }
 
   // Advance one step. Returns <0 while running, CELL_OK at tout, or a failure status.

@@ -27,6 +27,7 @@ enum CellStatus : int {
   CELL_ERRFAIL = 2,
   CELL_CONVFAIL = 3,
   CELL_BADINPUT = 4,
+  CELL_ADJ_STORAGE = 5,  // adjoint: checkpoint or tape capacity exceeded
 };
 
 struct BdfParams {
@@ -74,6 +75,19 @@ __device__ inline double sel(const double (&a)[K], int idx) {
   }
   return r;
 }
+
+// Operations on the Nordsieck array are reported to the accessor (rec_* members), so a recording
+// accessor can tape the step sequence for the discrete adjoint. NoTape is the empty default.
+struct NoTape {
+  __device__ void rec_rescale(int, double) const {}
+  __device__ void rec_pascal(int, double) const {}
+  __device__ void rec_order_up(int, int, int, double, const double*) const {}
+  __device__ void rec_order_down(int, const double*) const {}
+  __device__ void rec_correct(const struct BdfScalars&, bool, int) const {}
+  __device__ void rec_restart(double, double) const {}
+  __device__ void rec_init(double) const {}
+  __device__ void rec_mark_save() const {}
+};
 
 // Vector accessors declare kMemory = true when Z/A/E live in device memory (the batched form's
 // structure-of-arrays state). The Nordsieck updates below then run one component at a time and
@@ -158,6 +172,7 @@ __device__ inline void rescale(BdfScalars& S, const V& v) {
 #pragma unroll
   for (int j = 1; j <= BDF_QMAX; ++j) f[j] = f[j - 1] * S.eta;
   const int q = S.q;
+  v.rec_rescale(q, S.eta);
 #pragma unroll
   for (int i = 0; i < n; ++i) {
 #pragma unroll
@@ -173,6 +188,7 @@ __device__ inline void rescale(BdfScalars& S, const V& v) {
 // k = 1..q, j = q..k: z_{j-1} += sign z_j -- per component, in registers.
 template <int n, class V>
 __device__ inline void pascal(const V& v, int q, double sign) {
+  v.rec_pascal(q, sign);
 #pragma unroll
   for (int i = 0; i < n; ++i) {
     double z[BDF_QMAX + 1];
@@ -278,6 +294,7 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
     }
     const double A1 = (-alpha0 - alpha1) / prod;
     const int q = S.q, L = S.L, ia = S.indx_acor;
+    v.rec_order_up(q, L, ia, A1, S.l);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
       const double zl = A1 * v.Z(ia, i);
@@ -301,6 +318,7 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
       for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xi + S.l[i - 1] : S.l[i];
     }
     const int q = S.q;
+    v.rec_order_down(q, S.l);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
       const double zq = v.Z(q, i);
@@ -427,6 +445,7 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
   S.tau[1] = S.h;
   S.qwait--;
   const bool save_acor = S.qwait == 1 && S.q != qmax;
+  v.rec_correct(S, save_acor, qmax);
   {
     const int q = S.q;
 #pragma unroll
@@ -493,6 +512,7 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
       } else {
         S.eta = etaqp1;
         S.qprime = S.q + 1;
+        v.rec_mark_save();  // the correction just applied is also saved in column qmax
 #pragma unroll
         for (int i = 0; i < n; ++i) v.Z(qmax, i) = v.A(i);
       }

@@ -47,8 +47,11 @@ struct CellModel {
   }
 
   // ydot = f(t, y); src (may be null) is rhoY_src_k at src[k * sstride]. Returns the T used.
+  // MUL: reaction j's rate constant is multiplied by rmul[j] (the sensitivity/adjoint paths).
+  template <bool MUL = false>
   __device__ static double rhs(double t, const double* y, double rhoe0, double rhoesrc,
-                               const double* src, int64_t sstride, double* ydot) {
+                               const double* src, int64_t sstride, double* ydot,
+                               const double* rmul = nullptr) {
     double rho = 0.0;
 #pragma unroll
     for (int k = 0; k < N; ++k) rho += y[k];
@@ -61,7 +64,7 @@ struct CellModel {
     }
     const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, Y, y[N]);
     double wd[N];
-    Mech::wdot(C, T, wd);
+    Mech::template wdot<MUL>(C, T, wd, rmul);
     double ek[N], cvk[N];
     Mech::e_cv(T, ek, cvk);
     double num = rhoesrc, D = 0.0;
@@ -77,10 +80,13 @@ struct CellModel {
     return T;
   }
 
-  // Exact Jacobian of rhs() into J (n x n row-major, element type JT).
-  template <typename JT>
+  // Exact Jacobian of rhs() into J (n x n row-major, element type JT). fe (may be null) receives
+  // df/d(rho e0) = (df/dT at fixed rhoY) / (rho c_v): the column through which the initial energy
+  // (and the energy forcing) enters the RHS.
+  template <typename JT, bool MUL = false>
   __device__ static void jac(double t, const double* y, double rhoe0, double rhoesrc,
-                             const double* src, int64_t sstride, JT* J) {
+                             const double* src, int64_t sstride, JT* J, double* fe = nullptr,
+                             const double* rmul = nullptr) {
     double rho = 0.0;
 #pragma unroll
     for (int k = 0; k < N; ++k) rho += y[k];
@@ -93,7 +99,7 @@ struct CellModel {
     }
     const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, Y, y[N]);
     double wd[N], dwdT[N];
-    Mech::template wdot_jac<n, JT>(C, T, wd, J, dwdT);
+    Mech::template wdot_jac<n, JT, MUL>(C, T, wd, J, dwdT, rmul);
     double ek[N], cvk[N], dcv[N];
     Mech::e_cv(T, ek, cvk);
     Mech::dcv_dT(T, dcv);
@@ -128,6 +134,11 @@ struct CellModel {
     }
 #pragma unroll
     for (int k = 0; k < n; ++k) J[k * n + N] = JT(0);
+    if (fe) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) fe[k] = B[k] * iD;
+      fe[N] = dfTdT * iD;
+    }
   }
 };
 

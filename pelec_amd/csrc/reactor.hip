@@ -74,6 +74,10 @@ struct prc_reactor {
   int64_t last_iterations = 0;
   // PELEC_AMD_TRACE=1: one stderr line per graph replay of the batched form (live-cell counts)
   bool trace = false;
+  // adjoint: checkpoint segment length, checkpoints and tape records per cell; workspace
+  int adj_seg = 8, adj_kmax = 128, adj_tcap = 256;
+  double* aws = nullptr;
+  size_t aws_bytes = 0;
   // react_state workspace: rhoY (N), T, rho e, rho e forcing per cell of the box
   double* sws = nullptr;
   int64_t sws_cap = 0;
@@ -447,6 +451,9 @@ void release(prc_reactor* r) {
   if (r->sws) (void)hipFree(r->sws);
   r->sws = nullptr;
   r->sws_cap = 0;
+  if (r->aws) (void)hipFree(r->aws);
+  r->aws = nullptr;
+  r->aws_bytes = 0;
   r->graph_lu_fp32 = -1;
   r->ncu = 0;
   r->initialised = false;
@@ -554,6 +561,15 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "rk64.nsubsteps_max")) {
     if (v < 1) return PRC_EINVAL;
     r->rk_max = v;
+  } else if (!std::strcmp(key, "adj.checkpoint_steps")) {
+    if (v < 1) return PRC_EINVAL;
+    r->adj_seg = v;
+  } else if (!std::strcmp(key, "adj.max_checkpoints")) {
+    if (v < 1) return PRC_EINVAL;
+    r->adj_kmax = v;
+  } else if (!std::strcmp(key, "adj.segment_records")) {
+    if (v < 8) return PRC_EINVAL;
+    r->adj_tcap = v;
   } else if (!std::strcmp(key, "prc.kernel_timing")) {
     r->kernel_timing = v ? 1 : 0;
   } else {
@@ -801,6 +817,65 @@ int prc_react_state_box(prc_reactor* r, const int32_t lo[3], const int32_t hi[3]
   hipLaunchKernelGGL(k_state_unpack, dim3(nb), dim3(256), 0, s, sa);
   HIP_OK(hipGetLastError());
   *time += *dt_react;
+  return PRC_OK;
+}
+
+int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double* T0,
+                const double* rE0, const double* rYsrc, const double* rEsrc, double dt,
+                const double* logA, const double* ct_rY, const double* ct_T, double* rY_out,
+                double* T_out, double* g_rY0, double* g_rE0, double* g_logA, int32_t* status,
+                double* tape, int64_t tape_cap, int32_t* tape_count, void* stream) {
+  int e = ensure_init(r);
+  if (e) return e;
+  if (ncells < 0 || !(dt > 0.0) || r->integrator != 0) return PRC_EINVAL;
+  if (ncells == 0) return PRC_OK;
+  if (!rY0 || !T0 || !rE0 || !rY_out || !T_out || !g_rY0 || !g_rE0 || !g_logA || !status)
+    return PRC_EINVAL;
+  if (tape && (tape_cap < 1 || !tape_count)) return PRC_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t per_cell = (size_t)r->adj_kmax * CKW + (size_t)r->adj_tcap * TRS;
+  const size_t bytes = sizeof(double) * ((size_t)ncells * per_cell + (size_t)Mech::NRXN);
+  if (bytes > r->aws_bytes) {
+    if (r->aws) (void)hipFree(r->aws);
+    r->aws = nullptr;
+    r->aws_bytes = 0;
+    if (hipMalloc(&r->aws, bytes) != hipSuccess) return PRC_ENOMEM;
+    r->aws_bytes = bytes;
+  }
+  AdjArgs a{};
+  a.ncells = ncells;
+  a.rY0 = rY0;
+  a.T0 = T0;
+  a.rE0 = rE0;
+  a.rYsrc = rYsrc;
+  a.rEsrc = rEsrc;
+  a.dt = dt;
+  double* rmul = r->aws;
+  a.rmul = rmul;
+  a.ct_rY = ct_rY;
+  a.ct_T = ct_T;
+  a.rY_out = rY_out;
+  a.T_out = T_out;
+  a.g_rY0 = g_rY0;
+  a.g_rE0 = g_rE0;
+  a.g_p = g_logA;
+  a.status = status;
+  a.prm.rtol = r->rtol;
+  a.prm.atolv = r->d_atol;
+  a.prm.qmax = std::min(std::max(r->max_order, 1), BDF_QMAX);
+  a.prm.max_steps = r->max_steps;
+  a.seg = r->adj_seg;
+  a.kmax = r->adj_kmax;
+  a.ckpt = r->aws + Mech::NRXN;
+  a.tcap = r->adj_tcap;
+  a.tape = a.ckpt + (size_t)ncells * r->adj_kmax * CKW;
+  a.full_cap = tape ? tape_cap : 0;
+  a.full_tape = tape;
+  a.full_count = tape_count;
+  hipLaunchKernelGGL(k_exp, dim3((Mech::NRXN + 63) / 64), dim3(64), 0, s, (int)Mech::NRXN, logA,
+                     rmul);
+  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((ncells + 63) / 64)), dim3(64), 0, s, a);
+  HIP_OK(hipGetLastError());
   return PRC_OK;
 }
 

@@ -13,6 +13,7 @@
 #include "batched.h"
 #include "bdf_cell.h"
 #include "rk64.h"
+#include "adjoint.h"
 
 namespace {
 
@@ -235,7 +236,7 @@ __global__ void __launch_bounds__(64) react_rk64(const Args* __restrict__ ap) {
 
 // ------------------------------------------------------------------ batched work-queue form
 __device__ inline SoAV<NV> soa(const StateBuf& st, int64_t C, int64_t s) {
-  return SoAV<NV>{st.zn, st.acor, st.ewt, C, s};
+  return SoAV<NV>{{}, st.zn, st.acor, st.ewt, C, s};
 }
 
 __device__ inline void rhs_slot(const Args& a, const StateBuf& st, int64_t s, double t,
@@ -686,6 +687,178 @@ __global__ void k_compact_done(const Args* __restrict__ ap) {
 __global__ void k_init_counters(unsigned long long* c, int64_t nactive) {
   if (threadIdx.x < CT_COUNT)
     c[threadIdx.x] = (threadIdx.x == CT_NACTIVE) ? (unsigned long long)nactive : 0ull;
+}
+
+// ------------------------------------------------------------------ discrete adjoint
+struct AdjArgs {
+  int64_t ncells;
+  const double *rY0, *T0, *rE0, *rYsrc, *rEsrc;  // component-major, ncells per component
+  double dt;
+  const double* rmul;                             // NRXN rate multipliers exp(log A_j)
+  const double *ct_rY, *ct_T;                     // output cotangents (may be null)
+  double *rY_out, *T_out;                         // end state
+  double *g_rY0, *g_rE0, *g_p;                    // gradients; g_p is NRXN x ncells
+  int32_t* status;
+  BdfParams prm;
+  int seg;          // accepted steps per checkpoint segment
+  int kmax;         // checkpoints per cell
+  double* ckpt;     // ncells x kmax x CKW
+  int tcap;         // records per segment tape
+  double* tape;     // ncells x tcap x RS
+  int64_t full_cap; // optional complete forward tape (verification): records per cell
+  double* full_tape;
+  int32_t* full_count;
+};
+
+__global__ void k_exp(int n, const double* x, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x ? exp(x[i]) : 1.0;
+}
+
+using AdjCell = BdfCell<Mech, double, GTape<NV>, true, true>;
+constexpr int CKW = (BDF_QMAX + 1) * NV + BDF_SCALAR_WORDS + NV * NV;  // checkpoint words
+constexpr int TRS = GTape<NV>::RS;
+
+__device__ inline void ckpt_save(const AdjCell& cell, double* p) {
+  for (int j = 0; j <= BDF_QMAX; ++j)
+    for (int i = 0; i < NV; ++i) p[j * NV + i] = cell.zn[j][i];
+  uint64_t w[BDF_SCALAR_WORDS];
+  __builtin_memcpy(w, &cell.S, sizeof(BdfScalars));
+  double* q = p + (BDF_QMAX + 1) * NV;
+  for (int k = 0; k < BDF_SCALAR_WORDS; ++k) __builtin_memcpy(q + k, w + k, 8);
+  q += BDF_SCALAR_WORDS;
+  for (int k = 0; k < NV * NV; ++k) q[k] = cell.savedJ[k];
+}
+
+__device__ inline void ckpt_load(AdjCell& cell, const double* p) {
+  for (int j = 0; j <= BDF_QMAX; ++j)
+    for (int i = 0; i < NV; ++i) cell.zn[j][i] = p[j * NV + i];
+  uint64_t w[BDF_SCALAR_WORDS];
+  const double* q = p + (BDF_QMAX + 1) * NV;
+  for (int k = 0; k < BDF_SCALAR_WORDS; ++k) __builtin_memcpy(w + k, q + k, 8);
+  __builtin_memcpy(&cell.S, w, sizeof(BdfScalars));
+  q += BDF_SCALAR_WORDS;
+  for (int k = 0; k < NV * NV; ++k) cell.savedJ[k] = q[k];
+  // the Newton matrix in use at the checkpoint: I - gammap J, refactorised (deterministic)
+  for (int k = 0; k < NV * NV; ++k) cell.M[k] = -cell.S.gammap * cell.savedJ[k];
+  for (int k = 0; k < NV; ++k) cell.M[k * NV + k] += 1.0;
+  (void)lu_factor<NV, double>(cell.M, cell.piv);
+}
+
+// Forward integration with checkpoints, then the backward sweep segment by segment (re-running
+// each segment from its checkpoint with the tape on). One cell per lane.
+__global__ void __launch_bounds__(64) k_adjoint(const AdjArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  const int64_t nc = a.ncells;
+  AdjCell cell;
+  cell.rhoe0 = a.rE0[c];
+  cell.rhoesrc = a.rEsrc ? a.rEsrc[c] : 0.0;
+  cell.src = a.rYsrc ? a.rYsrc + c : nullptr;
+  cell.sstride = nc;
+  cell.prm = a.prm;
+  cell.rmul = a.rmul;
+  double y0[NV];
+  double rho = 0.0;
+  for (int k = 0; k < NSP; ++k) {
+    y0[k] = a.rY0[k * nc + c];
+    rho += y0[k];
+  }
+  y0[NSP] = a.T0[c];
+  auto fail = [&](int st) {
+    a.status[c] = st;
+    for (int k = 0; k < NSP; ++k) {
+      a.rY_out[k * nc + c] = y0[k];
+      a.g_rY0[k * nc + c] = 0.0;
+    }
+    a.T_out[c] = y0[NSP];
+    a.g_rE0[c] = 0.0;
+    for (int j = 0; j < Mech::NRXN; ++j) a.g_p[j * nc + c] = 0.0;
+  };
+  if (!(rho > 0.0) || !(y0[NSP] > 0.0) || !(a.dt > 0.0)) {
+    fail(CELL_BADINPUT);
+    return;
+  }
+  double* ck = a.ckpt + c * (int64_t)a.kmax * CKW;
+  // forward pass
+  if (a.full_tape) {
+    cell.tape.base = a.full_tape + c * a.full_cap * TRS;
+    cell.tape.cap = (int)a.full_cap;
+  }
+  cell.start(y0, a.dt);
+  int st;
+  do {
+    st = cell.advance();
+    if (st < 0 && cell.S.nst % a.seg == 0) {
+      const int k = cell.S.nst / a.seg;
+      if (k >= a.kmax) {
+        st = CELL_ADJ_STORAGE;
+        break;
+      }
+      ckpt_save(cell, ck + (int64_t)k * CKW);
+    }
+  } while (st < 0);
+  if (a.full_tape) a.full_count[c] = cell.tape.overflow ? -1 : cell.tape.count;
+  if (st != CELL_OK) {
+    fail(st);
+    return;
+  }
+  const int nst = cell.S.nst;
+  double yend[NV];
+  for (int i = 0; i < NV; ++i) yend[i] = cell.zn[0][i];
+  {
+    double Y[NSP];
+    double r2 = 0.0;
+    for (int k = 0; k < NSP; ++k) r2 += yend[k];
+    for (int k = 0; k < NSP; ++k) Y[k] = yend[k] / r2;
+    a.T_out[c] = Model::T_from_e((cell.rhoe0 + a.dt * cell.rhoesrc) / r2, Y, yend[NSP]);
+    for (int k = 0; k < NSP; ++k) a.rY_out[k * nc + c] = yend[k];
+  }
+  if (!a.ct_rY && !a.ct_T) {  // no cotangents: forward only
+    a.status[c] = CELL_OK;
+    for (int k = 0; k < NSP; ++k) a.g_rY0[k * nc + c] = 0.0;
+    a.g_rE0[c] = 0.0;
+    for (int j = 0; j < Mech::NRXN; ++j) a.g_p[j * nc + c] = 0.0;
+    return;
+  }
+  // backward pass
+  Adjoint<Mech> adj;
+  adj.rhoe0 = cell.rhoe0;
+  adj.rhoesrc = cell.rhoesrc;
+  adj.src = cell.src;
+  adj.sstride = nc;
+  adj.rmul = a.rmul;
+  adj.reset();
+  adj.seed(yend, a.dt, a.ct_rY ? a.ct_rY + c : nullptr, nc, a.ct_T ? a.ct_T[c] : 0.0);
+  double* tp = a.tape + c * (int64_t)a.tcap * TRS;
+  const int nseg = (nst + a.seg - 1) / a.seg;
+  for (int s = nseg - 1; s >= 0; --s) {
+    cell.tape.base = tp;
+    cell.tape.cap = a.tcap;
+    cell.tape.count = 0;
+    cell.tape.overflow = false;
+    if (s == 0) {
+      cell.start(y0, a.dt);  // records the initial z_1 = h0 f(0, y0)
+    } else {
+      ckpt_load(cell, ck + (int64_t)s * CKW);
+      cell.tstop = a.dt;
+    }
+    const int target = min((s + 1) * a.seg, nst);
+    while (cell.S.nst < target) {
+      const int r = cell.advance();
+      if (r >= 0 && r != CELL_OK) break;
+      if (r == CELL_OK) break;
+    }
+    if (cell.tape.overflow || cell.S.nst != target) {
+      fail(CELL_ADJ_STORAGE);
+      return;
+    }
+    for (int r = cell.tape.count - 1; r >= 0; --r) adj.back(tp + (int64_t)r * TRS);
+  }
+  a.status[c] = CELL_OK;
+  for (int k = 0; k < NSP; ++k) a.g_rY0[k * nc + c] = adj.gy0[k];
+  a.g_rE0[c] = adj.ge;
+  for (int j = 0; j < Mech::NRXN; ++j) a.g_p[j * nc + c] = adj.gp[j];
 }
 
 // ------------------------------------------------------------------ react_state pack / unpack

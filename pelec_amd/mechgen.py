@@ -329,20 +329,20 @@ class Gen:
                 w("      const double F = 1.0;")
                 if jac:
                     w("      const double g = 0.0, dlnF_T = 0.0;")
-            w("      const double kf = kinf * Pr * iPr1 * F;")
+            w(f"      const double kf = RMUL(kinf * Pr * iPr1 * F, {j});")
             if jac:
                 w(f"      const double dlnkinf = ({f(r.b)} + {f(r.Ea_R)} * invT) * invT;")
                 w(f"      const double dlnPr = ({f(b0 - r.b)} + {f(E0 - r.Ea_R)} * invT) * invT;")
-                w("      const double dkdM = kinf * k0r * F * (iPr1 * iPr1 + g * iPr1);")
+                w(f"      const double dkdM = RMUL(kinf * k0r * F * (iPr1 * iPr1 + g * iPr1), {j});")
                 w("      const double dlnk = dlnkinf + dlnPr * iPr1 + g * dlnPr + dlnF_T;")
         elif r.kind == "three-body":
-            w(f"      const double kb = {self.rate_expr(r.A, r.b, r.Ea_R)};")
+            w(f"      const double kb = RMUL({self.rate_expr(r.A, r.b, r.Ea_R)}, {j});")
             w(f"      const double kf = kb * M{self.effsets[tuple(r.efficiencies)]};")
             if jac:
                 w("      const double dkdM = kb;")
                 w(f"      const double dlnk = ({f(r.b)} + {f(r.Ea_R)} * invT) * invT;")
         else:
-            w(f"      const double kf = {self.rate_expr(r.A, r.b, r.Ea_R)};")
+            w(f"      const double kf = RMUL({self.rate_expr(r.A, r.b, r.Ea_R)}, {j});")
             if jac:
                 w(f"      const double dlnk = ({f(r.b)} + {f(r.Ea_R)} * invT) * invT;")
         w(f"      const double Pf = {self.prod_expr(r.reactants)};")
@@ -382,7 +382,9 @@ class Gen:
 
         # rates of progress
         w("  // rates of progress q_j [mol/(cm^3 s)]; scale (may be null) multiplies each q_j")
-        w("  __host__ __device__ static inline void progress(const double* C, double T, double* q) {")
+        w("  template <bool MUL = false>")
+        w("  __host__ __device__ static inline void progress(const double* C, double T, double* q, const double* rmul = nullptr) {")
+        w("    RMUL_DEF;")
         w("    const double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
@@ -396,7 +398,9 @@ class Gen:
         w("  }")
         # production rates
         w("  // molar production rates wdot_k [mol/(cm^3 s)]")
-        w("  __host__ __device__ static inline void wdot(const double* C, double T, double* wd) {")
+        w("  template <bool MUL = false>")
+        w("  __host__ __device__ static inline void wdot(const double* C, double T, double* wd, const double* rmul = nullptr) {")
+        w("    RMUL_DEF;")
         w("    double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
@@ -415,8 +419,9 @@ class Gen:
         # Jacobian
         ne = len(self.effsets)
         w("  // wdot and its Jacobian: dwdC[k*LD+m] = d wdot_k / d C_m, dwdT[k] = d wdot_k / dT")
-        w("  template <int LD, typename JT>")
-        w("  __host__ __device__ static inline void wdot_jac(const double* C, double T, double* wd, JT* dwdC, double* dwdT) {")
+        w("  template <int LD, typename JT, bool MUL = false>")
+        w("  __host__ __device__ static inline void wdot_jac(const double* C, double T, double* wd, JT* dwdC, double* dwdT, const double* rmul = nullptr) {")
+        w("    RMUL_DEF;")
         w("    double lnT = log(T), invT = 1.0 / T;")
         w(f"    const double refC = {f(M.P_ATM / M.R_ERG)} * invT, irefC = {f(M.R_ERG / M.P_ATM)} * T;")
         w("    (void)refC; (void)irefC;")
@@ -467,6 +472,13 @@ class Gen:
             self.emit_chain(j, "qj")
             w("    }")
         w("  }")
+        # transpose of the net stoichiometric matrix applied to a species vector
+        w("  // out[j] = sum_k nu_kj w[k] (net stoichiometric coefficients, products minus reactants)")
+        w("  __host__ __device__ static inline void nu_dot(const double* w, double* out) {")
+        for j, r in enumerate(self.m.reactions):
+            terms = [f"{f(v)} * w[{k}]" for k, v in sorted(self.nu_of(r).items()) if v != 0.0]
+            w(f"    out[{j}] = {' + '.join(terms) if terms else '0.0'};")
+        w("  }")
 
     # ------------------------------------------------------------------ top level
     def generate(self) -> str:
@@ -488,6 +500,12 @@ class Gen:
         w("#else")
         w("#define PELEC_REG_CHAIN(a, b, c, d) ((void)(d))")
         w("#endif")
+        w("#endif")
+        w("// RMUL(k, j): the rate constant of reaction j times its multiplier rmul[j] when MUL (the")
+        w("// per-reaction pre-exponential scaling exp(log A_j) of the sensitivity and adjoint paths)")
+        w("#ifndef RMUL")
+        w("#define RMUL_DEF (void)rmul")
+        w("#define RMUL(k, j) (MUL ? (k) * rmul[j] : (k))")
         w("#endif")
         w("#ifndef __HIPCC__")
         w("#define __host__")

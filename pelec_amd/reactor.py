@@ -83,6 +83,8 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "prc_compute_temp": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32), Array4,
                                         IArray4, ctypes.c_int, ctypes.c_void_p]),
+    "prc_adjoint": (ctypes.c_int, [ctypes.c_void_p, _i64] + [ctypes.c_void_p] * 5 + [ctypes.c_double]
+                    + [ctypes.c_void_p] * 10 + [_i64, ctypes.c_void_p, ctypes.c_void_p]),
     "prc_eos_pyt2re": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 6),
     "prc_eos_rey2t": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 5),
     "prc_eos_rty2hi": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 3),
@@ -278,6 +280,33 @@ class Reactor:
                                             ctypes.byref(dtc), ctypes.byref(tm),
                                             _stream_handle(stream)), "prc_react_state_box")
         return tm.value
+
+    def adjoint(self, rY0, T0, rE0, dt: float, logA=None, ct_rY=None, ct_T=None, rYsrc=None,
+                rEsrc=None, tape_cap: int = 0, stream=None):
+        """Discrete adjoint of the BDF reactor (prc_adjoint). rY0 (N, ncells), T0/rE0 (ncells,),
+        logA (NRXN,) or None, cotangents ct_rY (N, ncells) / ct_T (ncells,) or None: cuda float64.
+        Returns a dict: rY, T (end state), g_rY0, g_rE0, g_logA (NRXN, ncells), status, and with
+        tape_cap > 0 the complete forward tapes (tape, tape_count)."""
+        import torch
+        nc = T0.shape[0]
+        assert rY0.shape == (self.nsp, nc) and rY0.is_contiguous()
+        dev = T0.device
+        out = {"rY": torch.empty_like(rY0), "T": torch.empty_like(T0),
+               "g_rY0": torch.empty_like(rY0), "g_rE0": torch.empty_like(T0),
+               "g_logA": torch.empty((self.lib.prc_num_reactions(), nc), dtype=torch.float64, device=dev),
+               "status": torch.empty(nc, dtype=torch.int32, device=dev)}
+        tape = cnt = None
+        if tape_cap > 0:
+            tape = torch.empty((nc, tape_cap, 16 + self.nsp + 1), dtype=torch.float64, device=dev)
+            cnt = torch.empty(nc, dtype=torch.int32, device=dev)
+            out["tape"], out["tape_count"] = tape, cnt
+        _check(self.lib.prc_adjoint(self.h, nc, _ptr(rY0), _ptr(T0), _ptr(rE0), _ptr(rYsrc),
+                                    _ptr(rEsrc), float(dt), _ptr(logA), _ptr(ct_rY), _ptr(ct_T),
+                                    _ptr(out["rY"]), _ptr(out["T"]), _ptr(out["g_rY0"]),
+                                    _ptr(out["g_rE0"]), _ptr(out["g_logA"]), _ptr(out["status"]),
+                                    _ptr(tape), int(tape_cap), _ptr(cnt), _stream_handle(stream)),
+               "prc_adjoint")
+        return out
 
     def last_failures(self, stream=None) -> int:
         n = self.lib.prc_last_failures(self.h, _stream_handle(stream))

@@ -133,14 +133,20 @@ def test_react_batched_vs_oracle(lu_fp32, oracles):
     assert dT <= GATE_T and dY <= GATE_Y
 
 
-@pytest.mark.parametrize("wl,n,dt", [("lidryer_32", 2048, None), ("drm19_64", 512, 1e-6)])
-def test_react_rk64_vs_oracle(wl, n, dt, oracles):
-    """ReactorRK64 drop-in against the oracle's RK64 restatement (same tableau and controller)."""
+@pytest.mark.parametrize("wl,n,dt,rtol,atol", [("lidryer_32", 2048, None, None, None),
+                                               ("drm19_64", 512, 1e-6, 1e-8, 1e-12)])
+def test_react_rk64_vs_oracle(wl, n, dt, rtol, atol, oracles):
+    """ReactorRK64 drop-in against the oracle's RK64 restatement (same tableau and controller).
+    An explicit adaptive method's trajectories separate by about its truncation error when a
+    round-off difference flips one of its ~10^3 accept/reject decisions, so the drm19 case runs at
+    rtol 1e-8, where that error (2e-5 K against the Radau fixtures) is below the gate."""
     W = inputs.WORKLOADS[wl]
     dt = W.dt if dt is None else dt
+    rtol = W.rtol if rtol is None else rtol
+    atol = W.atol if atol is None else atol
     d = inputs.make_inputs(wl, n)
     r = Reactor(W.mech, chem_integrator="ReactorRK64")
-    r.configure(rtol=W.rtol, atol=W.atol)
+    r.configure(rtol=rtol, atol=atol)
     rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
     FC = torch.zeros_like(T)
     diag = torch.zeros((8, n), dtype=torch.int32, device=DEV)
@@ -150,7 +156,7 @@ def test_react_rk64_vs_oracle(wl, n, dt, oracles):
     torch.cuda.synchronize()
     assert r.last_failures() == 0
     r.close()
-    ref = oracles[W.mech].react_rk64(d["rY"], d["T"], d["rE"], dt, rtol=W.rtol, atol=W.atol,
+    ref = oracles[W.mech].react_rk64(d["rY"], d["T"], d["rE"], dt, rtol=rtol, atol=atol,
                                      nsub_guess=10, nsub_min=5, nsub_max=100000, nthreads=8)
     assert ref["nfail"] == 0
     out = {"rY": rY.cpu().numpy(), "T": T.cpu().numpy(), "rE": rE.cpu().numpy()}
