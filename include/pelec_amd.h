@@ -91,7 +91,20 @@ int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
  * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
  * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);
+/* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.blocks_newton,
+ * prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 = occupancy-derived), prc.compact_pct) */
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
+/* Read back an integer knob (the prc.* launch-shape knobs report the values in use). */
+int prc_get_param_int(prc_reactor* r, const char* key, int* value);
+/* Launch-shape autotuner of the batched form (prc.form 1, ReactorCvode): times react calls on the
+ * given sample of cells (device arrays, component-major; copied, not modified) over candidate
+ * shapes -- iterations per captured graph (prc.iters_per_graph), blocks per CU of the Newton,
+ * Jacobian and LU kernels (prc.blocks_newton / _jac / _lu), the compaction threshold
+ * (prc.compact_pct) -- by coordinate descent from the occupancy-derived defaults, keeps the
+ * fastest in the reactor and returns its time (ms) in *best_ms. Synchronises the stream. No
+ * counterpart in the reference (its GPU launch shapes are fixed at compile time). */
+int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double* T,
+                 const double* rE, double dt, double* best_ms, void* stream);
 /* Optional per-cell diagnostics (device int32 buffer, PRC_DIAG_NCOMP x capacity, component
  * stride `capacity`); cell c of the next react call writes diag[comp*capacity + c]. Pass NULL to
  * disable. */

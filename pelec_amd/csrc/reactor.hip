@@ -2,7 +2,7 @@
 // Built once per mechanism with -DPELEC_MECH=<ident> -DPELEC_MECH_HEADER="gen/mech_<name>.h".
 //
 // prc.form 0 (plain): one launch of react_plain per box; prc.form 1 (batched): k_start, then a
-// captured graph of ITERS_PER_GRAPH iterations replayed until every started cell is done, with a
+// captured graph of a fixed number of iterations replayed until every started cell is done, with a
 // compaction of the live slots whenever fewer than half remain alive. Each iteration forks into
 // two branches that run concurrently on disjoint cells: k_jac -> k_lu (set-up list) and k_newton
 // (Newton-phase cells). prc.lu_fp32 keeps the Newton-matrix factors in FP32 with FP64 refinement.
@@ -33,8 +33,18 @@ constexpr auto lu_kernel() {
   else
     return &k_lu<LUT>;
 }
-constexpr int ITERS_PER_GRAPH = 15;  // a multiple of NLISTS: every replay starts on list 0
-static_assert(ITERS_PER_GRAPH % NLISTS == 0, "replays must start on set-up list 0");
+// launch shape of the batched form (prc.* knobs; prc_autotune picks them by measurement)
+struct Shape {
+  int iters = 15;           // iterations per captured graph, a multiple of NLISTS
+  int bpc_newton = 0;       // blocks per CU of k_newton / k_jac / the LU kernel (0: occupancy)
+  int bpc_jac = 0;
+  int bpc_lu = 0;
+  int compact_pct = 50;     // compact when fewer than this percentage of the slots are alive
+  bool operator==(const Shape& o) const {
+    return iters == o.iters && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
+           bpc_lu == o.bpc_lu && compact_pct == o.compact_pct;
+  }
+};
 }
 
 struct prc_reactor {
@@ -69,6 +79,7 @@ struct prc_reactor {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_lu_fp32 = -1;
+  Shape shape, graph_shape;
   int ncu = 0;
   int occ_newton = 1, occ_jac = 1, occ_lu = 1;
   int64_t last_iterations = 0;
@@ -202,18 +213,18 @@ int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
 
 template <typename LUT>
 int capture_graph(prc_reactor* r) {
-  if (r->graph_exec && r->graph_lu_fp32 == r->lu_fp32) return PRC_OK;
+  if (r->graph_exec && r->graph_lu_fp32 == r->lu_fp32 && r->graph_shape == r->shape) return PRC_OK;
   if (r->graph_exec) {
     (void)hipGraphExecDestroy(r->graph_exec);
     r->graph_exec = nullptr;
   }
   hipStream_t s = r->cap_stream, s2 = r->cap_stream2;
   HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const int gj = r->ncu * r->occ_jac;
-  const int gl = r->ncu * r->occ_lu;
-  const int gn = r->ncu * r->occ_newton;
+  const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
+  const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
+  const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
   bool ok = true;
-  for (int it = 0; it < ITERS_PER_GRAPH && ok; ++it) {
+  for (int it = 0; it < r->shape.iters && ok; ++it) {
     const int par = it % NLISTS;
     // fork: Newton branch on s2, set-up branch on s; join before the next iteration
     ok = hipEventRecord(r->ev_fork, s) == hipSuccess &&
@@ -234,6 +245,7 @@ int capture_graph(prc_reactor* r) {
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return PRC_EHIP;
   r->graph_lu_fp32 = r->lu_fp32;
+  r->graph_shape = r->shape;
   return PRC_OK;
 }
 
@@ -252,22 +264,22 @@ int occupancy_setup(prc_reactor* r) {
   return PRC_OK;
 }
 
-// ITERS_PER_GRAPH iterations launched eagerly with the graph's fork/join structure and timing
+// One graph's worth of iterations launched eagerly with the graph's fork/join structure and timing
 // events around each kernel; the elapsed times are added to r->kt_ms after the replay's sync.
 template <typename LUT>
 int eager_iterations(prc_reactor* r, hipStream_t s) {
   constexpr int EV = 5;  // newton begin/end on s2; jac begin, jac end = lu begin, lu end on s
-  const size_t need = (size_t)EV * ITERS_PER_GRAPH;
+  const size_t need = (size_t)EV * r->shape.iters;
   while (r->kt_ev.size() < need) {
     hipEvent_t e = nullptr;
     HIP_OK(hipEventCreate(&e));
     r->kt_ev.push_back(e);
   }
   hipStream_t s2 = r->cap_stream2;
-  const int gj = r->ncu * r->occ_jac;
-  const int gl = r->ncu * r->occ_lu;
-  const int gn = r->ncu * r->occ_newton;
-  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
+  const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
+  const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
+  const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
+  for (int it = 0; it < r->shape.iters; ++it) {
     const int par = it % NLISTS;
     hipEvent_t* e = r->kt_ev.data() + EV * it;
     HIP_OK(hipEventRecord(r->ev_fork, s));
@@ -289,7 +301,7 @@ int eager_iterations(prc_reactor* r, hipStream_t s) {
 
 int eager_accumulate(prc_reactor* r) {
   constexpr int EV = 5;
-  for (int it = 0; it < ITERS_PER_GRAPH; ++it) {
+  for (int it = 0; it < r->shape.iters; ++it) {
     hipEvent_t* e = r->kt_ev.data() + EV * it;
     float t = 0.0f;
     HIP_OK(hipEventElapsedTime(&t, e[0], e[1]));
@@ -299,13 +311,13 @@ int eager_accumulate(prc_reactor* r) {
     HIP_OK(hipEventElapsedTime(&t, e[3], e[4]));
     r->kt_ms[2] += t;
   }
-  r->kt_launches += ITERS_PER_GRAPH;
+  r->kt_launches += r->shape.iters;
   return PRC_OK;
 }
 
 template <typename LUT>
 int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
-  if (r->graph_lu_fp32 != r->lu_fp32 || !r->ncu) {
+  if (r->graph_lu_fp32 != r->lu_fp32 || !r->ncu || !(r->graph_shape == r->shape)) {
     const int e = occupancy_setup<LUT>(r);
     if (e) return e;
   }
@@ -318,7 +330,7 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
   HIP_OK(hipStreamSynchronize(s));
   int e = capture_graph<LUT>(r);
   if (e) return e;
-  const int gn = r->ncu * r->occ_newton;
+  const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
   int64_t iters = 0;
   // safety net: every cell ends within max_steps steps of a bounded number of attempts
   const int64_t max_iters = (int64_t)(r->max_steps + 1) * 64 + 64;
@@ -332,7 +344,7 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
     } else {
       HIP_OK(hipGraphLaunch(r->graph_exec, s));
     }
-    iters += ITERS_PER_GRAPH;
+    iters += r->shape.iters;
     HIP_OK(hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -349,7 +361,7 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
       std::fprintf(stderr, "[pelec_amd] iter %lld started %lld done %lld alive %lld slots %lld\n",
                    (long long)iters, (long long)started, (long long)done, (long long)alive,
                    (long long)nactive);
-    if (alive * 2 < nactive && nactive > 4096) {
+    if (alive * 100 < nactive * r->shape.compact_pct && nactive > 4096) {
       hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, r->d_args);
       hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, r->d_args, 0);
       hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, r->d_args);
@@ -570,11 +582,47 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "adj.segment_records")) {
     if (v < 8) return PRC_EINVAL;
     r->adj_tcap = v;
+  } else if (!std::strcmp(key, "prc.iters_per_graph")) {
+    if (v < NLISTS || v > 300 || v % NLISTS) return PRC_EINVAL;
+    r->shape.iters = v;
+  } else if (!std::strcmp(key, "prc.blocks_newton")) {
+    if (v < 0 || v > 64) return PRC_EINVAL;
+    r->shape.bpc_newton = v;
+  } else if (!std::strcmp(key, "prc.blocks_jac")) {
+    if (v < 0 || v > 64) return PRC_EINVAL;
+    r->shape.bpc_jac = v;
+  } else if (!std::strcmp(key, "prc.blocks_lu")) {
+    if (v < 0 || v > 64) return PRC_EINVAL;
+    r->shape.bpc_lu = v;
+  } else if (!std::strcmp(key, "prc.compact_pct")) {
+    if (v < 1 || v > 100) return PRC_EINVAL;
+    r->shape.compact_pct = v;
   } else if (!std::strcmp(key, "prc.kernel_timing")) {
     r->kernel_timing = v ? 1 : 0;
   } else {
     return PRC_EINVAL;
   }
+  return PRC_OK;
+}
+
+int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
+  if (!r || !key || !v) return PRC_EINVAL;
+  if (!std::strcmp(key, "prc.iters_per_graph"))
+    *v = r->shape.iters;
+  else if (!std::strcmp(key, "prc.blocks_newton"))
+    *v = r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton;
+  else if (!std::strcmp(key, "prc.blocks_jac"))
+    *v = r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac;
+  else if (!std::strcmp(key, "prc.blocks_lu"))
+    *v = r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu;
+  else if (!std::strcmp(key, "prc.compact_pct"))
+    *v = r->shape.compact_pct;
+  else if (!std::strcmp(key, "prc.form"))
+    *v = r->form;
+  else if (!std::strcmp(key, "prc.lu_fp32"))
+    *v = r->lu_fp32;
+  else
+    return PRC_EINVAL;
   return PRC_OK;
 }
 
@@ -730,6 +778,95 @@ int64_t prc_last_failures(prc_reactor* r, void* stream) {
     return PRC_EHIP;
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return PRC_EHIP;
   return (int64_t)r->h_ctr[CT_FAILED];
+}
+
+int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double* T,
+                 const double* rE, double dt, double* best_ms, void* stream) {
+  int e = ensure_init(r);
+  if (e) return e;
+  if (r->form != 1 || r->integrator != 0 || ncells <= 0 || !rY || !T || !rE || !(dt > 0.0))
+    return PRC_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = sizeof(double) * (size_t)(NSP + 2) * ncells;
+  double* work = nullptr;
+  if (hipMalloc(&work, bytes) != hipSuccess) return PRC_ENOMEM;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipFree(work);
+    if (e0) (void)hipEventDestroy(e0);
+    return PRC_EHIP;
+  }
+  // the caller's diagnostics buffer is left alone while tuning
+  int32_t* diag = r->diag;
+  const int64_t diag_cap = r->diag_cap;
+  r->diag = nullptr;
+  r->diag_cap = 0;
+  double* wY = work;
+  double* wT = work + (size_t)NSP * ncells;
+  double* wE = wT + ncells;
+  // one timed react call of the sample with the current shape (best of two)
+  auto measure = [&](double& ms) -> int {
+    ms = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIP_OK(hipMemcpyAsync(wY, rY, sizeof(double) * NSP * ncells, hipMemcpyDeviceToDevice, s));
+      HIP_OK(hipMemcpyAsync(wT, T, sizeof(double) * ncells, hipMemcpyDeviceToDevice, s));
+      HIP_OK(hipMemcpyAsync(wE, rE, sizeof(double) * ncells, hipMemcpyDeviceToDevice, s));
+      double d = dt, tm = 0.0;
+      HIP_OK(hipEventRecord(e0, s));
+      const int rc = prc_react_flat(r, wY, nullptr, wT, wE, nullptr, nullptr, nullptr, ncells, &d,
+                                    &tm, s);
+      if (rc) return rc;
+      HIP_OK(hipEventRecord(e1, s));
+      HIP_OK(hipEventSynchronize(e1));
+      float t = 0.0f;
+      HIP_OK(hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, (double)t);
+    }
+    return PRC_OK;
+  };
+  int rc = PRC_OK;
+  double best = 0.0;
+  {
+    // occupancy-derived defaults are the starting point
+    Shape base = r->shape;
+    if (!r->ncu) rc = r->lu_fp32 ? occupancy_setup<float>(r) : occupancy_setup<double>(r);
+    if (!rc) rc = measure(best);
+    Shape cur = r->shape;
+    const int on = r->occ_newton, oj = r->occ_jac, ol = r->occ_lu;
+    struct Axis {
+      int Shape::*field;
+      int cand[3];
+    };
+    const Axis axes[] = {{&Shape::iters, {6, 15, 30}},
+                         {&Shape::bpc_newton, {on, 2 * on, 4 * on}},
+                         {&Shape::bpc_jac, {oj, 2 * oj, 4 * oj}},
+                         {&Shape::bpc_lu, {ol, 2 * ol, 4 * ol}},
+                         {&Shape::compact_pct, {25, 50, 75}}};
+    for (const Axis& ax : axes) {
+      if (rc) break;
+      for (int c : ax.cand) {
+        Shape trial = cur;
+        trial.*ax.field = std::min(c, ax.field == &Shape::iters ? 300 : 64);
+        if (trial == cur) continue;
+        r->shape = trial;
+        double ms = 0.0;
+        rc = measure(ms);
+        if (rc) break;
+        if (ms < best) {
+          best = ms;
+          cur = trial;
+        }
+      }
+    }
+    r->shape = rc ? base : cur;
+  }
+  r->diag = diag;
+  r->diag_cap = diag_cap;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(work);
+  if (best_ms) *best_ms = best;
+  return rc;
 }
 
 int prc_kernel_times(prc_reactor* r, double* ms, int64_t* launches) {

@@ -61,6 +61,10 @@ _SIGS = {
     "prc_set_param_real": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]),
     "prc_set_param_int": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "prc_set_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _i64]),
+    "prc_get_param_int": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "prc_autotune": (ctypes.c_int, [ctypes.c_void_p, _i64, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "prc_close": (ctypes.c_int, [ctypes.c_void_p]),
     "prc_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "prc_react_box": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
@@ -207,6 +211,26 @@ class Reactor:
             self.set_param("rk64.nsubsteps_min", lo)
             self.set_param("rk64.nsubsteps_max", hi)
         return self
+
+    SHAPE_KEYS = ("prc.iters_per_graph", "prc.blocks_newton", "prc.blocks_jac", "prc.blocks_lu",
+                  "prc.compact_pct")
+
+    def get_param(self, key: str) -> int:
+        v = ctypes.c_int(0)
+        _check(self.lib.prc_get_param_int(self.h, key.encode(), ctypes.byref(v)), f"get {key}")
+        return v.value
+
+    def shape(self):
+        """Launch shape of the batched form in use."""
+        return {k: self.get_param(k) for k in self.SHAPE_KEYS}
+
+    def autotune(self, rY, T, rE, dt: float, stream=None):
+        """Pick the batched launch shape by timing react calls on a sample (rY (N, n), T, rE (n,)
+        cuda float64; not modified). Returns (shape dict, best ms)."""
+        ms = ctypes.c_double(0.0)
+        _check(self.lib.prc_autotune(self.h, T.shape[0], _ptr(rY), _ptr(T), _ptr(rE), float(dt),
+                                     ctypes.byref(ms), _stream_handle(stream)), "prc_autotune")
+        return self.shape(), ms.value
 
     def set_diagnostics(self, diag_tensor):
         """diag_tensor: int32 cuda tensor shaped (8, capacity) or None."""

@@ -386,3 +386,27 @@ def test_compute_temp_vs_oracle():
     np.testing.assert_allclose(g[6], ref[6], rtol=1e-11)
     assert g[6, 9] == S[6, 9]
     assert np.array_equal(np.delete(g, 6, 0), np.delete(S, 6, 0))
+
+
+def test_autotune_shape_and_parity(oracles):
+    """prc_autotune picks a launch shape by timing the batched form on a sample; the tuned shape
+    is valid and the reactor still matches the oracle with it."""
+    W = inputs.WORKLOADS["drm19_64"]
+    n = 4096
+    d = inputs.make_inputs("drm19_64", n)
+    r = Reactor(W.mech)
+    r.configure(rtol=W.rtol, atol=W.atol, form="batched", lu_fp32=True)
+    note("autotune drm19 4096 cells")
+    shape, ms = r.autotune(to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"]), W.dt)
+    note(f"  tuned shape {shape}, {ms:.2f} ms")
+    assert ms > 0.0 and shape["prc.iters_per_graph"] % 3 == 0
+    assert all(shape[k] >= 1 for k in ("prc.blocks_newton", "prc.blocks_jac", "prc.blocks_lu"))
+    rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+    r.react_flat(rY, T, rE, W.dt)
+    torch.cuda.synchronize()
+    assert r.last_failures() == 0
+    r.close()
+    ref = oracles[W.mech].react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol,
+                                nthreads=8)
+    dT, dY, _ = compare({"rY": rY.cpu().numpy(), "T": T.cpu().numpy()}, ref)
+    assert dT <= GATE_T and dY <= GATE_Y
