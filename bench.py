@@ -44,6 +44,48 @@ def measured_traffic(workload):
     return None
 
 
+def time_adjoint(torch, Reactor, inputs):
+    """Discrete adjoint leg: drm19, 16384 cells, dt = 1e-4 s; one call = forward with checkpoints
+    plus the backward sweep, for the loss sum w T + sum v Y_OH (w, v ~ N(0, 1))."""
+    wl = inputs.WORKLOADS["drm19_adj"]
+    d = inputs.make_inputs(wl.name)
+    dev = "cuda:0"
+    n = wl.ncells
+    rY0 = torch.tensor(d["rY"], dtype=torch.float64, device=dev)
+    T0 = torch.tensor(d["T"], dtype=torch.float64, device=dev)
+    rE0 = torch.tensor(d["rE"], dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(4004)
+    w = torch.tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
+    v = torch.tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
+    r = Reactor(wl.mech)
+    r.configure(rtol=wl.rtol, atol=wl.atol)
+    fwd = r.adjoint(rY0, T0, rE0, wl.dt)          # end state for the loss cotangents
+    oh = r.species.index("OH")
+    rho = fwd["rY"].sum(0)
+    ct_rY = (-(v * fwd["rY"][oh] / rho**2))[None, :].repeat(r.nsp, 1)
+    ct_rY[oh] += v / rho
+    ct_rY = ct_rY.contiguous()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(2):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        out = r.adjoint(rY0, T0, rE0, wl.dt, ct_rY=ct_rY, ct_T=w)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    nfail = int((out["status"] != 0).sum())
+    g = out["g_logA"].sum(1)
+    r.close()
+    t = min(times)
+    log(f"adjoint: {t * 1e3:.1f} ms for {n} cells, failures {nfail}")
+    return {"workload": wl.name, "ncells": n, "dt": wl.dt, "ms_per_call": t * 1e3,
+            "cells_per_s": n / t, "failed_cells": nfail,
+            "max_abs_grad_logA": float(g.abs().max()),
+            "note": "forward with checkpoints every 8 steps + backward sweep with recomputation"}
+
+
 def host_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -52,10 +94,13 @@ def host_threads():
     return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
 
 
-def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timing=False):
+def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timing=False,
+              shape=None):
     r = Reactor(wl.mech)
     r.configure(rtol=wl.rtol, atol=wl.atol, form=form, lu_fp32=lu_fp32,
                 kernel_timing=kernel_timing)
+    for k, v in (shape or {}).items():
+        r.set_param(k, int(v))
     n = d0["T"].shape[0]
     rY, T, rE = d0["rY"].clone(), d0["T"].clone(), d0["rE"].clone()
     FC = torch.zeros_like(T)
@@ -106,6 +151,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "plain"], default=None,
                     help="time one form only (profiling runs)")
+    ap.add_argument("--no-tune", action="store_true", help="occupancy-derived launch shape")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -125,8 +172,17 @@ def main():
     n = wl.ncells
 
     head = plain = None
+    shape = None
+    if args.only != "plain" and not args.no_tune:
+        # launch shape of the batched form picked by the autotuner on this workload (untimed)
+        tuner = Reactor(wl.mech)
+        tuner.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=True)
+        shape, tune_ms = tuner.autotune(d0["rY"], d0["T"], d0["rE"], wl.dt)
+        tuner.close()
+        log(f"autotuned shape {shape} ({tune_ms:.1f} ms on the workload)")
     if args.only != "plain":
-        head = time_form(torch, Reactor, wl, d0, "batched", True, args.steps, args.warmup)
+        head = time_form(torch, Reactor, wl, d0, "batched", True, args.steps, args.warmup,
+                         shape=shape)
     if args.only != "batched":
         plain = time_form(torch, Reactor, wl, d0, "plain", False, args.steps, args.warmup)
     if args.only:
@@ -138,7 +194,7 @@ def main():
     # per-kernel device time of the headline form: one extra call with HIP events around every
     # k_newton / k_jac / k_lu launch (eager launches; the graph replays of the timed calls are
     # the same kernels on the same streams)
-    kt = time_form(torch, Reactor, wl, d0, "batched", True, 1, 0, kernel_timing=True)
+    kt = time_form(torch, Reactor, wl, d0, "batched", True, 1, 0, kernel_timing=True, shape=shape)
     t_head = float(np.median(head["times"]))
     t_plain = float(np.median(plain["times"]))
     value = n / t_head
@@ -178,6 +234,10 @@ def main():
     dgp = plain["diag"].astype(np.float64)
     Wp = flops.work(wl.mech, dgp[2].sum(), dgp[3].sum(), dgp[4].sum(), dgp[5].sum(), dgp[1].sum())
 
+    adj = None
+    if not args.no_adjoint:
+        adj = time_adjoint(torch, Reactor, inputs)
+
     cpu = None
     if not args.no_cpu:
         # bounded CPU baseline: oracle on host cores, sample grown until ~cpu-seconds of work
@@ -212,7 +272,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "mechanism": wl.mech, "ncells": n, "dt": wl.dt,
                    "rtol": wl.rtol, "atol": wl.atol,
-                   "form": "batched work-queue, FP32 Newton-matrix LU, FP64 elsewhere"},
+                   "form": "batched work-queue, FP32 Newton-matrix LU, FP64 elsewhere",
+                   "launch_shape": shape},
         "speedup_vs_plain_fp64": t_plain / t_head,
         "plain_fp64": {"value": value_plain, "ms_per_step": t_plain * 1e3,
                        "achieved_TFLOPs": Wp / t_plain / 1e12},
@@ -226,6 +287,7 @@ def main():
                      "whole_call_TFLOPs": achieved_call / 1e12,
                      "cells_steps_mean": float(dg[1].mean()), "cells_steps_max": int(dg[1].max())},
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

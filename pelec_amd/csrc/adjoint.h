@@ -166,8 +166,9 @@ struct Adjoint {
   double gp[R];
   double ge;
   double gy0[n];
-  // scratch
-  double J[n * n];
+  // scratch: J and Mf point to n x n blocks of device memory (per cell)
+  double* J;
+  double* Mf;
   double fe[n];
   int piv[n];
 
@@ -276,7 +277,6 @@ struct Adjoint {
           nu[i] = mu;
         }
         jacobian(t, y);
-        double Mf[n * n];
         for (int k = 0; k < n * n; ++k) Mf[k] = -gamma * J[k];
         for (int k = 0; k < n; ++k) Mf[k * n + k] += 1.0;
         (void)lu_factor<n, double>(Mf, piv);

@@ -3,6 +3,8 @@
 // straightforward GPU form of ReactorCvode::react (Source/React.cpp:222-228) that the batched
 // form is measured against; both use the step logic of bdf_core.h.
 #pragma once
+#include <type_traits>
+
 #include "bdf_core.h"
 
 namespace pelec {
@@ -80,8 +82,10 @@ struct NoTapeRec {
 
 // POLISH: after the modified Newton iteration converges, the corrector equation is solved to
 // machine precision with the exact Jacobian (the discrete adjoint linearises the exactly solved
-// step map, so its forward trajectory must be that map's).
-template <class Mech, typename LUT, class Tape = NoTapeRec, bool MUL = false, bool POLISH = false>
+// step map, so its forward trajectory must be that map's). EXTMAT: the n x n matrices live in
+// caller-provided device memory (savedJ, M, Jx point there) instead of private memory.
+template <class Mech, typename LUT, class Tape = NoTapeRec, bool MUL = false, bool POLISH = false,
+          bool EXTMAT = false>
 struct BdfCell {
   using Model = CellModel<Mech>;
   static constexpr int N = Mech::NSP;
@@ -97,12 +101,13 @@ struct BdfCell {
   BdfScalars S;
   double zn[BDF_QMAX + 1][n];
   double ewt[n], acor[n];
-  double savedJ[n * n];
-  LUT M[n * n];
+  std::conditional_t<EXTMAT, double*, double[n * n]> savedJ;
+  std::conditional_t<EXTMAT, LUT*, LUT[n * n]> M;
   int piv[n];
   Tape tape;
   const double* rmul = nullptr;  // per-reaction rate multipliers (MUL)
-  double Jx[POLISH ? n * n : 1];  // exact Jacobian / Newton matrix of the polish
+  // exact Jacobian / Newton matrix of the polish
+  std::conditional_t<EXTMAT, double*, double[POLISH ? n * n : 1]> Jx;
   int pivx[POLISH ? n : 1];
 
   struct Vec {

@@ -846,7 +846,8 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
       if (rc) break;
       for (int c : ax.cand) {
         Shape trial = cur;
-        trial.*ax.field = std::min(c, ax.field == &Shape::iters ? 300 : 64);
+        const bool blocks = ax.field != &Shape::iters && ax.field != &Shape::compact_pct;
+        trial.*ax.field = blocks ? std::min(c, 64) : c;
         if (trial == cur) continue;
         r->shape = trial;
         double ms = 0.0;
@@ -970,7 +971,8 @@ int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double*
     return PRC_EINVAL;
   if (tape && (tape_cap < 1 || !tape_count)) return PRC_EINVAL;
   hipStream_t s = (hipStream_t)stream;
-  const size_t per_cell = (size_t)r->adj_kmax * CKW + (size_t)r->adj_tcap * TRS;
+  const size_t per_cell =
+      (size_t)r->adj_kmax * CKW + (size_t)r->adj_tcap * TRS + (size_t)ADJ_MATS * NV * NV;
   const size_t bytes = sizeof(double) * ((size_t)ncells * per_cell + (size_t)Mech::NRXN);
   if (bytes > r->aws_bytes) {
     if (r->aws) (void)hipFree(r->aws);
@@ -1006,6 +1008,7 @@ int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double*
   a.ckpt = r->aws + Mech::NRXN;
   a.tcap = r->adj_tcap;
   a.tape = a.ckpt + (size_t)ncells * r->adj_kmax * CKW;
+  a.mats = a.tape + (size_t)ncells * r->adj_tcap * TRS;
   a.full_cap = tape ? tape_cap : 0;
   a.full_tape = tape;
   a.full_count = tape_count;

@@ -708,6 +708,7 @@ struct AdjArgs {
   int64_t full_cap; // optional complete forward tape (verification): records per cell
   double* full_tape;
   int32_t* full_count;
+  double* mats;     // ncells x 5 x n x n: the integrator's and the adjoint's matrices
 };
 
 __global__ void k_exp(int n, const double* x, double* y) {
@@ -715,7 +716,8 @@ __global__ void k_exp(int n, const double* x, double* y) {
   if (i < n) y[i] = x ? exp(x[i]) : 1.0;
 }
 
-using AdjCell = BdfCell<Mech, double, GTape<NV>, true, true>;
+using AdjCell = BdfCell<Mech, double, GTape<NV>, true, true, true>;
+constexpr int ADJ_MATS = 5;  // savedJ, M, Jx (forward) and J, Mf (backward) per cell
 constexpr int CKW = (BDF_QMAX + 1) * NV + BDF_SCALAR_WORDS + NV * NV;  // checkpoint words
 constexpr int TRS = GTape<NV>::RS;
 
@@ -751,7 +753,11 @@ __global__ void __launch_bounds__(64) k_adjoint(const AdjArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncells) return;
   const int64_t nc = a.ncells;
+  double* mats = a.mats + c * (int64_t)ADJ_MATS * NV * NV;
   AdjCell cell;
+  cell.savedJ = mats;
+  cell.M = mats + NV * NV;
+  cell.Jx = mats + 2 * NV * NV;
   cell.rhoe0 = a.rE0[c];
   cell.rhoesrc = a.rEsrc ? a.rEsrc[c] : 0.0;
   cell.src = a.rYsrc ? a.rYsrc + c : nullptr;
@@ -828,6 +834,8 @@ __global__ void __launch_bounds__(64) k_adjoint(const AdjArgs a) {
   adj.src = cell.src;
   adj.sstride = nc;
   adj.rmul = a.rmul;
+  adj.J = mats + 3 * NV * NV;
+  adj.Mf = mats + 4 * NV * NV;
   adj.reset();
   adj.seed(yend, a.dt, a.ct_rY ? a.ct_rY + c : nullptr, nc, a.ct_T ? a.ct_T[c] : 0.0);
   double* tp = a.tape + c * (int64_t)a.tcap * TRS;
