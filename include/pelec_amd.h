@@ -91,14 +91,17 @@ int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
  * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
  * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);
-/* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.blocks_newton,
- * prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 = occupancy-derived), prc.compact_pct) */
+/* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.cooperative (bit 0:
+ * the Newton kernel, bit 1: the Jacobian kernel run one cell per 32-lane group instead of one
+ * cell per lane), prc.blocks_newton, prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 =
+ * occupancy-derived), prc.compact_pct) */
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
 /* Read back an integer knob (the prc.* launch-shape knobs report the values in use). */
 int prc_get_param_int(prc_reactor* r, const char* key, int* value);
 /* Launch-shape autotuner of the batched form (prc.form 1, ReactorCvode): times react calls on the
  * given sample of cells (device arrays, component-major; copied, not modified) over candidate
- * shapes -- iterations per captured graph (prc.iters_per_graph), blocks per CU of the Newton,
+ * shapes -- the kernel forms (prc.cooperative), iterations per captured graph
+ * (prc.iters_per_graph), blocks per CU of the Newton,
  * Jacobian and LU kernels (prc.blocks_newton / _jac / _lu), the compaction threshold
  * (prc.compact_pct) -- by coordinate descent from the occupancy-derived defaults, keeps the
  * fastest in the reactor and returns its time (ms) in *best_ms. Synchronises the stream. No
@@ -217,6 +220,18 @@ int prc_eos_rty2hi(int64_t ncells, const double* T, double* hi, void* stream);
 /* Production rates (mechanism productionRate / RTY2WDOT): wdot_k [mol/(cm^3 s)] from rho, T, Y. */
 int prc_eos_rty2wdot(int64_t ncells, const double* rho, const double* T, const double* Y,
                      double* wdot, void* stream);
+
+/* ---- reactor model ------------------------------------------------------------------------ */
+/* The constant-volume rho*e reactor's RHS f(t, y) and exact Jacobian df/dy at given states: the
+ * CVODE user functions of ReactorCvode (cF_RHS and the analytic Jacobian of ReactorCvodeUtils /
+ * ReactorCvodeJacobian, CMake/BuildPelePhysicsLib.cmake:132-141). y = [rhoY_0..N-1, T] and ydot
+ * component-major (n x ncells, n = N+1); rhoe0 per cell; rhoesrc (rho e forcing) and rYsrc
+ * (N x ncells) may be null; J (ncells x n x n, row-major per cell) may be null. cooperative = 0
+ * evaluates one cell per lane, 1 one cell per 32-lane group (the kinetics of the cooperative
+ * batched kernels). Device pointers. */
+int prc_model_eval(int64_t ncells, double t, const double* y, const double* rhoe0,
+                   const double* rhoesrc, const double* rYsrc, double* ydot, double* J,
+                   int cooperative, void* stream);
 
 #ifdef __cplusplus
 }

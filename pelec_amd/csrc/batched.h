@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bdf_core.h"
+#include "group_model.h"
 
 #ifndef PELEC_MEMORY_FENCE
 #define PELEC_MEMORY_FENCE() __asm__ __volatile__("" ::: "memory")
@@ -66,13 +67,42 @@ struct StateBuf {
 };
 
 template <int n>
-struct SoAV : NoTape {
+struct SoAV : NoTape, LaneComps {
   static constexpr bool kMemory = true;
   double* zn;
   double* ac;
   double* ew;
   int64_t C;
   int64_t s;
+  __device__ double& Z(int j, int i) const { return zn[((int64_t)j * n + i) * C + s]; }
+  __device__ double& A(int i) const { return ac[(int64_t)i * C + s]; }
+  __device__ double& E(int i) const { return ew[(int64_t)i * C + s]; }
+};
+
+// One lane's share of a group-distributed vector (the work vectors of a group accessor).
+struct LaneVal {
+  double x = 0.0;
+  __device__ double& operator[](int) { return x; }
+  __device__ const double& operator[](int) const { return x; }
+};
+
+// Group accessor: component i of slot s on lane i of a G-lane group, in the structure-of-arrays
+// storage of SoAV. The integrator scalars are replicated on every lane of the group.
+template <int n, int G>
+struct GrpV : NoTape {
+  static constexpr bool kMemory = true;
+  static constexpr int kStep = G;
+  double* zn;
+  double* ac;
+  double* ew;
+  int64_t C;
+  int64_t s;
+  int l;  // lane within the group = the component this lane holds
+  __device__ int c0() const { return l; }
+  __device__ static double sum(double x) { return Group<G>::sum(x); }
+  __device__ static double max(double x) { return Group<G>::max(x); }
+  template <int m>
+  using Arr = LaneVal;
   __device__ double& Z(int j, int i) const { return zn[((int64_t)j * n + i) * C + s]; }
   __device__ double& A(int i) const { return ac[(int64_t)i * C + s]; }
   __device__ double& E(int i) const { return ew[(int64_t)i * C + s]; }
@@ -89,6 +119,24 @@ __device__ inline void store_scalars(const BdfScalars& S, uint64_t* sc, int64_t 
   __builtin_memcpy(w, &S, sizeof(S));
 #pragma unroll
   for (int k = 0; k < BDF_SCALAR_WORDS; ++k) sc[k * C + s] = w[k];
+}
+
+// The integrator scalars of slot s moved between structure-of-arrays storage and a copy shared by
+// a G-lane group (in LDS): lane l moves words l, l + G, ... (the group's lanes together load
+// or store the record once, instead of every lane holding the whole record in registers).
+template <int G>
+__device__ inline void load_scalars_group(BdfScalars& S, const uint64_t* sc, int64_t C, int64_t s,
+                                          int l) {
+  uint64_t* w = reinterpret_cast<uint64_t*>(&S);
+  for (int k = l; k < BDF_SCALAR_WORDS; k += G) w[k] = sc[k * C + s];
+  Group<G>::sync();
+}
+template <int G>
+__device__ inline void store_scalars_group(const BdfScalars& S, uint64_t* sc, int64_t C,
+                                           int64_t s, int l) {
+  Group<G>::sync();
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(&S);
+  for (int k = l; k < BDF_SCALAR_WORDS; k += G) sc[k * C + s] = w[k];
 }
 
 // Wave-aggregated append of `val` for the lanes with pred set: one atomic per wave, lane order
@@ -212,6 +260,27 @@ __device__ inline int lu_rows(T (&row)[n], int r, int& pos) {
     }
   }
   return ier;
+}
+
+// x = (LU)^{-1} P b by a G-lane group: lane l holds b_l and row l of the factors in pivot order
+// (as k_lu_rows stores them: unit-lower multipliers, U, inverted diagonal) and perm_l = perm[l].
+// Forward and back substitution broadcast one solved component per step. Returns x_l (0 on lanes
+// >= n).
+template <int n, typename T, int G>
+__device__ inline double lu_solve_group(const T (&row)[n], int perm_l, double b, int l) {
+  T x = T(__shfl(b, perm_l, G));
+#pragma unroll
+  for (int j = 0; j < n - 1; ++j) {
+    const T xj = Group<G>::bcast(x, j);
+    if (l > j && l < n) x -= row[j] * xj;
+  }
+#pragma unroll
+  for (int j = n - 1; j >= 0; --j) {
+    if (l == j) x *= row[j];
+    const T xj = Group<G>::bcast(x, j);
+    if (l < j) x -= row[j] * xj;
+  }
+  return l < n ? double(x) : 0.0;
 }
 
 // x = (LU)^{-1} P b with factors from lu_lds (AoS block `lu`, inverted diagonal). b in/out (FP64),

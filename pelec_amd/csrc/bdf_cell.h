@@ -110,7 +110,7 @@ struct BdfCell {
   std::conditional_t<EXTMAT, double*, double[POLISH ? n * n : 1]> Jx;
   int pivx[POLISH ? n : 1];
 
-  struct Vec {
+  struct Vec : LaneComps {
     static constexpr bool kMemory = false;
     BdfCell* c;
     __device__ double& Z(int j, int i) const { return c->zn[j][i]; }
@@ -131,7 +131,7 @@ struct BdfCell {
     __device__ void rec_init(double h) const { c->tape.rec_init(*this, h); }
     __device__ void rec_mark_save() const { c->tape.rec_mark_save(); }
   };
-  __device__ Vec vec() { return Vec{this}; }
+  __device__ Vec vec() { return Vec{{}, this}; }
 
   __device__ void f(double t, const double* yy, double* out) {
     S.nfe++;

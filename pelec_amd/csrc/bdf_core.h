@@ -89,6 +89,20 @@ struct NoTape {
   __device__ void rec_mark_save() const {}
 };
 
+// Component distribution of a vector accessor. Per-lane accessors (LaneComps) hold all n
+// components of a cell in one lane: component loops run i = 0..n-1, sums are plain, and work
+// vectors are private arrays. A group accessor holds component i on lane i of a group of lanes:
+// c0() is the lane, kStep the group size, sum()/max() are group reductions returning the same
+// value on every lane, and Arr<n> is one register per lane.
+struct LaneComps {
+  static constexpr int kStep = 1;
+  __device__ static constexpr int c0() { return 0; }
+  __device__ static double sum(double x) { return x; }
+  __device__ static double max(double x) { return x; }
+  template <int m>
+  using Arr = double[m];
+};
+
 // Vector accessors declare kMemory = true when Z/A/E live in device memory (the batched form's
 // structure-of-arrays state). The Nordsieck updates below then run one component at a time and
 // end each component with a compiler memory fence: the compiler keeps one component's <= 6
@@ -141,28 +155,28 @@ template <int n, class V>
 __device__ inline double wrms_acor(const V& v) {
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < n; ++i) {
+  for (int i = v.c0(); i < n; i += V::kStep) {
     const double w = v.A(i) * v.E(i);
     s += w * w;
   }
-  return sqrt(s / n);
+  return sqrt(v.sum(s) / n);
 }
 
-template <int n, class V>
-__device__ inline double wrms_reg(const V& v, const double* x) {
+template <int n, class V, class X>
+__device__ inline double wrms_reg(const V& v, const X& x) {
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < n; ++i) {
+  for (int i = v.c0(); i < n; i += V::kStep) {
     const double w = x[i] * v.E(i);
     s += w * w;
   }
-  return sqrt(s / n);
+  return sqrt(v.sum(s) / n);
 }
 
 template <int n, class V>
 __device__ inline void set_ewt(const V& v, const BdfParams& prm) {
 #pragma unroll
-  for (int i = 0; i < n; ++i) v.E(i) = 1.0 / (prm.rtol * fabs(v.Z(0, i)) + prm.atolv[i]);
+  for (int i = v.c0(); i < n; i += V::kStep) v.E(i) = 1.0 / (prm.rtol * fabs(v.Z(0, i)) + prm.atolv[i]);
 }
 
 template <int n, class V>
@@ -174,7 +188,7 @@ __device__ inline void rescale(BdfScalars& S, const V& v) {
   const int q = S.q;
   v.rec_rescale(q, S.eta);
 #pragma unroll
-  for (int i = 0; i < n; ++i) {
+  for (int i = v.c0(); i < n; i += V::kStep) {
 #pragma unroll
     for (int j = 1; j <= BDF_QMAX; ++j)
       if (j <= q) v.Z(j, i) *= f[j];
@@ -190,7 +204,7 @@ template <int n, class V>
 __device__ inline void pascal(const V& v, int q, double sign) {
   v.rec_pascal(q, sign);
 #pragma unroll
-  for (int i = 0; i < n; ++i) {
+  for (int i = v.c0(); i < n; i += V::kStep) {
     double z[BDF_QMAX + 1];
     load_hist(v, i, q, z);
 #pragma unroll
@@ -296,7 +310,7 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
     const int q = S.q, L = S.L, ia = S.indx_acor;
     v.rec_order_up(q, L, ia, A1, S.l);
 #pragma unroll
-    for (int i = 0; i < n; ++i) {
+    for (int i = v.c0(); i < n; i += V::kStep) {
       const double zl = A1 * v.Z(ia, i);
       v.Z(L, i) = zl;
 #pragma unroll
@@ -320,7 +334,7 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
     const int q = S.q;
     v.rec_order_down(q, S.l);
 #pragma unroll
-    for (int i = 0; i < n; ++i) {
+    for (int i = v.c0(); i < n; i += V::kStep) {
       const double zq = v.Z(q, i);
 #pragma unroll
       for (int j = 2; j < BDF_QMAX; ++j)
@@ -449,7 +463,7 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
   {
     const int q = S.q;
 #pragma unroll
-    for (int i = 0; i < n; ++i) {
+    for (int i = v.c0(); i < n; i += V::kStep) {
       const double ai = v.A(i);
 #pragma unroll
       for (int j = 0; j <= BDF_QMAX; ++j)
@@ -480,11 +494,11 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
       if (S.q > 1) {
         double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < n; ++i) {
+        for (int i = v.c0(); i < n; i += V::kStep) {
           const double w = v.Z(S.q, i) * v.E(i);
           s += w * w;
         }
-        const double ddn = sqrt(s / n) * S.tq[1];
+        const double ddn = sqrt(v.sum(s) / n) * S.tq[1];
         etaqm1 = 1.0 / (pow(cv::BIAS1 * ddn, 1.0 / S.q) + cv::ADDON);
       }
       double etaqp1 = 0.0;
@@ -492,11 +506,11 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
         const double cquot = (S.tq[5] / S.saved_tq5) * pow(S.h / S.tau[2], (double)S.L);
         double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < n; ++i) {
+        for (int i = v.c0(); i < n; i += V::kStep) {
           const double w = (v.A(i) - cquot * v.Z(qmax, i)) * v.E(i);
           s += w * w;
         }
-        const double dup = sqrt(s / n) * S.tq[3];
+        const double dup = sqrt(v.sum(s) / n) * S.tq[3];
         etaqp1 = 1.0 / (pow(cv::BIAS3 * dup, 1.0 / (S.L + 1)) + cv::ADDON);
       }
       const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
@@ -514,7 +528,7 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
         S.qprime = S.q + 1;
         v.rec_mark_save();  // the correction just applied is also saved in column qmax
 #pragma unroll
-        for (int i = 0; i < n; ++i) v.Z(qmax, i) = v.A(i);
+        for (int i = v.c0(); i < n; i += V::kStep) v.Z(qmax, i) = v.A(i);
       }
       set_eta(S);
     }
@@ -543,22 +557,23 @@ __device__ inline double hin(BdfScalars& S, const V& v, double tout, F&& f) {
   const double hlb = 100.0 * tround;
   double hub_inv = 0.0;
 #pragma unroll
-  for (int i = 0; i < n; ++i) {
+  for (int i = v.c0(); i < n; i += V::kStep) {
     const double d = 0.1 * fabs(v.Z(0, i)) + 1.0 / v.E(i);
     hub_inv = fmax(hub_inv, fabs(v.Z(1, i)) / d);
   }
+  hub_inv = v.max(hub_inv);
   double hub = 0.1 * tdist;
   if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
   double hg = sqrt(hlb * hub);
   if (hub < hlb) return hg;
   double hnew = hg;
-  double y[n], t[n];
+  typename V::template Arr<n> y, t;
   for (int count1 = 1; count1 <= 4; ++count1) {
 #pragma unroll
-    for (int i = 0; i < n; ++i) y[i] = v.Z(0, i) + hg * v.Z(1, i);
+    for (int i = v.c0(); i < n; i += V::kStep) y[i] = v.Z(0, i) + hg * v.Z(1, i);
     f(S.tn + hg, y, t);
 #pragma unroll
-    for (int i = 0; i < n; ++i) t[i] = (t[i] - v.Z(1, i)) / hg;
+    for (int i = v.c0(); i < n; i += V::kStep) t[i] = (t[i] - v.Z(1, i)) / hg;
     const double yddnrm = wrms_reg<n>(v, t);
     hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
     if (count1 == 4) break;
@@ -576,22 +591,22 @@ __device__ inline double hin(BdfScalars& S, const V& v, double tout, F&& f) {
 // One Newton correction (after f(tn, y) is in ftemp): tempv = M^{-1}(gamma f - rl1 zn1 - acor),
 // scaled by 2/(1+gamrat); acor += tempv. Returns 0 converged, 1 continue, 2 failed (retry with a
 // fresh Jacobian when it is stale), 3 failed with a current Jacobian. The solve is a callable.
-template <int n, class V, class Solve>
-__device__ inline int newton_correct(BdfScalars& S, const V& v, const double* ftemp, Solve&& solve) {
-  double tempv[n];
+template <int n, class V, class FT, class Solve>
+__device__ inline int newton_correct(BdfScalars& S, const V& v, const FT& ftemp, Solve&& solve) {
+  typename V::template Arr<n> tempv;
   const double rl1 = 1.0 / S.l[1];
 #pragma unroll
-  for (int i = 0; i < n; ++i) tempv[i] = S.gamma * ftemp[i] - (rl1 * v.Z(1, i) + v.A(i));
+  for (int i = v.c0(); i < n; i += V::kStep) tempv[i] = S.gamma * ftemp[i] - (rl1 * v.Z(1, i) + v.A(i));
   solve(tempv);
   if (S.gamrat != 1.0) {
     const double sc = 2.0 / (1.0 + S.gamrat);
 #pragma unroll
-    for (int i = 0; i < n; ++i) tempv[i] *= sc;
+    for (int i = v.c0(); i < n; i += V::kStep) tempv[i] *= sc;
   }
   S.nni++;
   const double del = wrms_reg<n>(v, tempv);
 #pragma unroll
-  for (int i = 0; i < n; ++i) v.A(i) += tempv[i];
+  for (int i = v.c0(); i < n; i += V::kStep) v.A(i) += tempv[i];
   if (S.m > 0) S.crate = fmax(cv::CRDOWN * S.crate, del / S.delp);
   const double dcon = del * fmin(1.0, S.crate) / S.tq[4];
   if (dcon <= 1.0) {

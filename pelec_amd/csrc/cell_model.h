@@ -28,6 +28,12 @@ struct CellModel {
   __device__ static double T_from_e(double e, const double* Y, double T) {
     double pe[Mech::NTMID * 12];
     Mech::mix_e_poly(Y, pe);
+    return T_from_pe(e, pe, T);
+  }
+
+  // The Newton iteration itself, from the mixture polynomial pe[(g*2 + b)*6 + i] (Tmid group g,
+  // branch b = 0 below / 1 above Tmid, coefficient i of e(T)).
+  __device__ static double T_from_pe(double e, const double* pe, double T) {
     for (int it = 0; it < 20; ++it) {
       double fv = -e, dv = 0.0;
 #pragma unroll

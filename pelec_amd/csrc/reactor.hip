@@ -36,15 +36,30 @@ constexpr auto lu_kernel() {
 // launch shape of the batched form (prc.* knobs; prc_autotune picks them by measurement)
 struct Shape {
   int iters = 15;           // iterations per captured graph, a multiple of NLISTS
+  int coop = 0;             // bit 0: k_newton_g, bit 1: k_jac_g (one cell per 32-lane group)
   int bpc_newton = 0;       // blocks per CU of k_newton / k_jac / the LU kernel (0: occupancy)
   int bpc_jac = 0;
   int bpc_lu = 0;
   int compact_pct = 50;     // compact when fewer than this percentage of the slots are alive
   bool operator==(const Shape& o) const {
-    return iters == o.iters && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
+    return iters == o.iters && coop == o.coop && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
            bpc_lu == o.bpc_lu && compact_pct == o.compact_pct;
   }
 };
+
+// The Newton and Jacobian kernels of one batched iteration: one cell per lane, or one cell per
+// 32-lane group (two per 64-lane block)
+using PhaseKernel = void (*)(const Args*, int);
+struct PhaseKernels {
+  PhaseKernel newton, jac;
+};
+template <typename LUT>
+PhaseKernels phase_kernels(const Shape& sh) {
+  PhaseKernels k{&k_newton<LUT>, &k_jac};
+  if (sh.coop & 1) k.newton = &k_newton_g<LUT, 64>;
+  if (sh.coop & 2) k.jac = &k_jac_g<64>;
+  return k;
+}
 }
 
 struct prc_reactor {
@@ -223,6 +238,7 @@ int capture_graph(prc_reactor* r) {
   const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
   const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
   const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
+  const PhaseKernels pk = phase_kernels<LUT>(r->shape);
   bool ok = true;
   for (int it = 0; it < r->shape.iters && ok; ++it) {
     const int par = it % NLISTS;
@@ -230,8 +246,8 @@ int capture_graph(prc_reactor* r) {
     ok = hipEventRecord(r->ev_fork, s) == hipSuccess &&
          hipStreamWaitEvent(s2, r->ev_fork, 0) == hipSuccess;
     if (!ok) break;
-    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s2, r->d_args, par);
-    hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(pk.newton, dim3(gn), dim3(64), 0, s2, r->d_args, par);
+    hipLaunchKernelGGL(pk.jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
     hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, r->d_args, par);
     ok = hipEventRecord(r->ev_join, s2) == hipSuccess &&
          hipStreamWaitEvent(s, r->ev_join, 0) == hipSuccess;
@@ -255,9 +271,10 @@ int occupancy_setup(prc_reactor* r) {
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipDeviceGetAttribute(&r->ncu, hipDeviceAttributeMultiprocessorCount, dev));
   int o = 0;
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_newton<LUT>, 64, 0));
+  const PhaseKernels pk = phase_kernels<LUT>(r->shape);
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, pk.newton, 64, 0));
   r->occ_newton = std::max(o, 1);
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_jac, 64, 0));
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, pk.jac, 64, 0));
   r->occ_jac = std::max(o, 1);
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lu_kernel<LUT>(), 64, 0));
   r->occ_lu = std::max(o, 1);
@@ -279,16 +296,17 @@ int eager_iterations(prc_reactor* r, hipStream_t s) {
   const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
   const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
   const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
+  const PhaseKernels pk = phase_kernels<LUT>(r->shape);
   for (int it = 0; it < r->shape.iters; ++it) {
     const int par = it % NLISTS;
     hipEvent_t* e = r->kt_ev.data() + EV * it;
     HIP_OK(hipEventRecord(r->ev_fork, s));
     HIP_OK(hipStreamWaitEvent(s2, r->ev_fork, 0));
     HIP_OK(hipEventRecord(e[0], s2));
-    hipLaunchKernelGGL(k_newton<LUT>, dim3(gn), dim3(64), 0, s2, r->d_args, par);
+    hipLaunchKernelGGL(pk.newton, dim3(gn), dim3(64), 0, s2, r->d_args, par);
     HIP_OK(hipEventRecord(e[1], s2));
     HIP_OK(hipEventRecord(e[2], s));
-    hipLaunchKernelGGL(k_jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(pk.jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
     HIP_OK(hipEventRecord(e[3], s));
     hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, r->d_args, par);
     HIP_OK(hipEventRecord(e[4], s));
@@ -585,6 +603,9 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "prc.iters_per_graph")) {
     if (v < NLISTS || v > 300 || v % NLISTS) return PRC_EINVAL;
     r->shape.iters = v;
+  } else if (!std::strcmp(key, "prc.cooperative")) {
+    if (v < 0 || v > 3) return PRC_EINVAL;
+    r->shape.coop = v;
   } else if (!std::strcmp(key, "prc.blocks_newton")) {
     if (v < 0 || v > 64) return PRC_EINVAL;
     r->shape.bpc_newton = v;
@@ -617,6 +638,8 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
     *v = r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu;
   else if (!std::strcmp(key, "prc.compact_pct"))
     *v = r->shape.compact_pct;
+  else if (!std::strcmp(key, "prc.cooperative"))
+    *v = r->shape.coop;
   else if (!std::strcmp(key, "prc.form"))
     *v = r->form;
   else if (!std::strcmp(key, "prc.lu_fp32"))
@@ -832,6 +855,22 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
     if (!r->ncu) rc = r->lu_fp32 ? occupancy_setup<float>(r) : occupancy_setup<double>(r);
     if (!rc) rc = measure(best);
     Shape cur = r->shape;
+    // kernel pair first (each one cell per lane or per group), at occupancy-derived block counts
+    for (int c = 0; c <= 3 && !rc; ++c) {
+      Shape trial = cur;
+      trial.coop = c;
+      trial.bpc_newton = trial.bpc_jac = trial.bpc_lu = 0;
+      if (trial == cur) continue;
+      r->shape = trial;
+      double ms = 0.0;
+      rc = measure(ms);
+      if (!rc && ms < best) {
+        best = ms;
+        cur = trial;
+      }
+    }
+    r->shape = cur;
+    if (!rc) rc = r->lu_fp32 ? occupancy_setup<float>(r) : occupancy_setup<double>(r);
     const int on = r->occ_newton, oj = r->occ_jac, ol = r->occ_lu;
     struct Axis {
       int Shape::*field;
@@ -1051,6 +1090,22 @@ int prc_eos_rty2wdot(int64_t nc, const double* rho, const double* T, const doubl
                      double* wdot, void* stream) {
   if (nc < 0 || (nc > 0 && (!rho || !T || !Y || !wdot))) return PRC_EINVAL;
   return launch_1d(eos_rty2wdot, nc, stream, rho, T, Y, wdot);
+}
+
+int prc_model_eval(int64_t nc, double t, const double* y, const double* rhoe0,
+                   const double* rhoesrc, const double* rYsrc, double* ydot, double* J,
+                   int cooperative, void* stream) {
+  if (nc < 0 || (nc > 0 && (!y || !rhoe0 || !ydot))) return PRC_EINVAL;
+  if (nc == 0) return PRC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (cooperative)
+    hipLaunchKernelGGL(k_model_group, dim3((unsigned)((nc + 1) / 2)), dim3(64), 0, s, nc, t, y,
+                       rhoe0, rhoesrc, rYsrc, ydot, J);
+  else
+    hipLaunchKernelGGL(k_model_lane, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, s, nc, t, y,
+                       rhoe0, rhoesrc, rYsrc, ydot, J);
+  HIP_OK(hipGetLastError());
+  return PRC_OK;
 }
 
 }  // extern "C"

@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(64) react_rk64(const Args* __restrict__ ap) {
 
 // ------------------------------------------------------------------ batched work-queue form
 __device__ inline SoAV<NV> soa(const StateBuf& st, int64_t C, int64_t s) {
-  return SoAV<NV>{{}, st.zn, st.acor, st.ewt, C, s};
+  return SoAV<NV>{{}, {}, st.zn, st.acor, st.ewt, C, s};
 }
 
 __device__ inline void rhs_slot(const Args& a, const StateBuf& st, int64_t s, double t,
@@ -618,6 +618,214 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup,
                 (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done);
+  }
+}
+
+// ------------------------------------------------------------------ cooperative kernels
+// One cell per 32-lane group (two per wave): lane k holds state component k, the species and
+// reaction work of the RHS and Jacobian is spread over the group (group_model.h), and the CVODE
+// step logic runs on the group accessor (component k on lane k, the integrator scalars replicated,
+// norms as group reductions). Same phases, lists, slot storage and counters as the one-cell-per-lane
+// kernels, which they replace one for one (prc.cooperative).
+constexpr int GRP = 32;
+using GModel = GroupModel<Mech, GRP>;
+
+// store_cell for a group: lane k < N writes rhoY_k, T(e_end, Y) by the group's T(e).
+__device__ inline void store_cell_g(const Args& a, int64_t g, int status, const Counters& S,
+                                    double yl, int l, double rhoe0, double rhoesrc, double* sh) {
+  int64_t loc;
+  const BoxDev& b = box_of(a, g, loc);
+  int i, j, k;
+  cell_ijk(b, loc, i, j, k);
+  if (status == CELL_OK) {
+    const double rho = GModel::Grp::sum(l < NSP ? yl : 0.0);
+    const double irho = 1.0 / rho;
+    const double rEn = rhoe0 + a.dt * rhoesrc;
+    const double Tg = GModel::Grp::bcast(yl, NSP);
+    const double Tn = GModel::T_from_e(rEn * irho, l < NSP ? yl * irho : 0.0, l, Tg, sh);
+    if (l < NSP) b.rY.at(i, j, k, l) = yl;
+    if (l == 0) {
+      b.T.at(i, j, k, 0) = Tn;
+      b.rE.at(i, j, k, 0) = rEn;
+    }
+  }
+  if (l == 0) {
+    if (b.FC.p) b.FC.at(i, j, k, 0) = (double)S.nfe;
+    write_diag(a, g, status, &S);
+  }
+}
+
+// Jacobians of the set-up list entries that need one, one cell per group (k_jac's work).
+template <int BS>
+__global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int par) {
+  constexpr int CPB = BS / GRP;
+  __shared__ double lds[CPB * GModel::JAC_WORDS];
+  const Args& a = *ap;
+  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  const int l = threadIdx.x & (GRP - 1);
+  double* sh = lds + (threadIdx.x / GRP) * GModel::JAC_WORDS;
+  for (int64_t base = (int64_t)blockIdx.x * CPB; base < cnt; base += (int64_t)gridDim.x * CPB) {
+    const int64_t idx = base + threadIdx.x / GRP;
+    if (idx >= cnt) continue;
+    const int64_t s = a.list[par][idx];
+    bool need;
+    double tn;
+    {
+      BdfScalars S;
+      load_scalars(S, st.sc, a.C, s);
+      need = jac_needed(S);
+      if (need) {
+        S.nje++;
+        S.nstlj = S.nst;
+        S.jcur = 1;
+      } else {
+        S.jcur = 0;
+      }
+      tn = S.tn;
+      if (l == 0) store_scalars(S, st.sc, a.C, s);
+    }
+    if (need) {
+      const int64_t g = st.cell[s];
+      const double yl = l < NV ? st.zn[(int64_t)l * a.C + s] : 0.0;
+      const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
+      GModel::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + g * (int64_t)(NV * NV));
+    }
+  }
+}
+
+// One Newton correction per Newton-phase slot, one cell per group (k_newton's work).
+template <typename LUT, int BS>
+__global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, int par) {
+  constexpr int CPB = BS / GRP;
+  __shared__ double lds[CPB * GModel::RHS_WORDS];
+  // the integrator scalars of each cell live in LDS, one copy per group (in registers they would
+  // be replicated on every lane of the group and more than halve the occupancy)
+  __shared__ BdfScalars scl[CPB];
+  const Args& a = *ap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
+  const int l = threadIdx.x & (GRP - 1);
+  double* sh = lds + (threadIdx.x / GRP) * GModel::RHS_WORDS;
+  for (int64_t base = (int64_t)blockIdx.x * CPB; base < nact; base += (int64_t)gridDim.x * CPB) {
+    const int64_t s = base + threadIdx.x / GRP;
+    const int ph = s < nact ? st.phase[s] : PH_DONE;
+    const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
+    bool need_setup = false, done = false;
+    if (act) {
+      const GrpV<NV, GRP> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
+      const int64_t g = st.cell[s];
+      const double rhoe0 = st.rhoe[s], rhoesrc = st.rhoe[a.C + s];
+      const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
+      constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
+      constexpr int W_M = offsetof(BdfScalars, m) / 8;
+      constexpr int SH_M = (offsetof(BdfScalars, m) % 8) * 8;
+      LaneVal ftemp;
+      {
+        // f(tn, zn[0] + acor) before the integrator scalars are loaded (register budget)
+        double tn;
+        const uint64_t w = st.sc[W_TN * a.C + s];
+        __builtin_memcpy(&tn, &w, 8);
+        const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
+        const double yl = l < NV ? v.Z(0, l) + (m0 > 0 ? v.A(l) : 0.0) : 0.0;
+        ftemp.x = GModel::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
+      }
+      BdfScalars& S = scl[threadIdx.x / GRP];
+      load_scalars_group<GRP>(S, st.sc, a.C, s, l);
+      S.nfe++;
+      if (S.m == 0 && l < NV) v.A(l) = 0.0;
+      const double gammap = S.gammap;
+      auto solve = [&](LaneVal& b) {
+        LUT row[NV];
+        int pl = l;
+        if (l < NV) {
+          const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV) + l * NV;
+#pragma unroll
+          for (int c = 0; c < NV; ++c) row[c] = lr[c];
+          pl = a.perm[g * NV + l];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) row[c] = LUT(0);
+        }
+        if constexpr (sizeof(LUT) == sizeof(double)) {
+          b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
+        } else {
+          // FP32 factors, one FP64 refinement step against the FP64 Jacobian (k_newton's)
+          const double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
+          double jx = 0.0;
+          const double* Jr = a.J + g * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
+#pragma unroll
+          for (int c = 0; c < NV; ++c) {
+            const double xc = GModel::Grp::bcast(x, c);
+            if (l < NV) jx += Jr[c] * xc;
+          }
+          const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
+          b.x = x + lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
+        }
+      };
+      const int r = newton_correct<NV>(S, v, ftemp, solve);
+      int status = -1;
+      if (r == 2) {
+        S.convfail = 2;
+        S.m = 0;
+        need_setup = true;
+      } else if (r != 1) {
+        bool attempt = false;
+        if (r == 3) {
+          const int e = conv_fail<NV>(S, v);
+          if (e >= 0)
+            status = e;
+          else
+            attempt = true;
+        } else {
+          const double dsm = S.acnrm * S.tq[2];
+          if (dsm <= 1.0) {
+            finish_step<NV>(S, v, dsm, a.prm.qmax);
+            if (reached_tstop(S, a.dt)) {
+              status = CELL_OK;
+            } else if (S.nst >= a.prm.max_steps) {
+              status = CELL_MAXSTEPS;
+            } else {
+              set_ewt<NV>(v, a.prm);
+              step_prologue<NV>(S, v, a.prm.qmax);
+              attempt = true;
+            }
+          } else {
+            const int e = error_test_fail<NV>(S, v, dsm, a.prm.qmax);
+            if (e >= 0) {
+              status = e;
+            } else {
+              if (e == -2) {
+                const double yl = l < NV ? v.Z(0, l) : 0.0;
+                S.nfe++;
+                const double fl = GModel::rhs(S.tn, yl, l, rhoe0, rhoesrc, srcl, sh);
+                if (l < NV) v.Z(1, l) = S.h * fl;
+              }
+              attempt = true;
+            }
+          }
+        }
+        if (attempt) need_setup = begin_attempt<NV>(S, v, a.dt);
+      }
+      if (status >= 0) {
+        done = true;
+        if (l == 0) {
+          st.phase[s] = PH_DONE;
+          if (status != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
+        }
+        store_cell_g(a, g, status, counters(S), l < NV ? v.Z(0, l) : 0.0, l, rhoe0, rhoesrc, sh);
+      } else if (l == 0) {
+        if (need_setup)
+          st.phase[s] = PH_SETUP;
+        else if (ph != PH_NEWTON)
+          st.phase[s] = PH_NEWTON;
+      }
+      store_scalars_group<GRP>(S, st.sc, a.C, s, l);
+    }
+    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup && l == 0,
+                (int32_t)s);
+    wave_count(&a.ctr[CT_DONE], done && l == 0);
   }
 }
 
@@ -1056,6 +1264,40 @@ __global__ void eos_rty2wdot(int64_t nc, const double* rho, const double* T, con
   Mech::wdot(C, T[c], wd);
 #pragma unroll
   for (int s = 0; s < NSP; ++s) wdot[s * nc + c] = wd[s];
+}
+
+// Reactor model at given states (prc_model_eval): y and ydot component-major (n x nc), J per cell
+// n x n row-major. One cell per lane (CellModel) ...
+__global__ void __launch_bounds__(64) k_model_lane(int64_t nc, double t, const double* y,
+                                                    const double* rhoe0, const double* rhoesrc,
+                                                    const double* src, double* ydot, double* J) {
+  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (c >= nc) return;
+  double yy[NV], f[NV];
+#pragma unroll
+  for (int e = 0; e < NV; ++e) yy[e] = y[e * nc + c];
+  const double r1 = rhoesrc ? rhoesrc[c] : 0.0;
+  Model::rhs(t, yy, rhoe0[c], r1, src ? src + c : nullptr, nc, f);
+#pragma unroll
+  for (int e = 0; e < NV; ++e) ydot[e * nc + c] = f[e];
+  if (J) Model::template jac<double>(t, yy, rhoe0[c], r1, src ? src + c : nullptr, nc, J + c * (int64_t)(NV * NV));
+}
+
+// ... or one cell per 32-lane group (GroupModel), two per block.
+__global__ void __launch_bounds__(64) k_model_group(int64_t nc, double t, const double* y,
+                                                     const double* rhoe0, const double* rhoesrc,
+                                                     const double* src, double* ydot, double* J) {
+  __shared__ double lds[2 * GModel::JAC_WORDS];
+  const int64_t c = (int64_t)blockIdx.x * 2 + threadIdx.x / GRP;
+  const int l = threadIdx.x & (GRP - 1);
+  if (c >= nc) return;
+  double* sh = lds + (threadIdx.x / GRP) * GModel::JAC_WORDS;
+  const double yl = l < NV ? y[l * nc + c] : 0.0;
+  const double srcl = (src && l < NSP) ? src[l * nc + c] : 0.0;
+  const double r1 = rhoesrc ? rhoesrc[c] : 0.0;
+  const double f = GModel::rhs(t, yl, l, rhoe0[c], r1, srcl, sh);
+  if (l < NV) ydot[l * nc + c] = f;
+  if (J) GModel::jac(t, yl, l, rhoe0[c], r1, srcl, sh, J + c * (int64_t)(NV * NV));
 }
 
 }  // namespace

@@ -17,6 +17,11 @@ The emitted header is written for gfx950 one-cell-per-lane execution:
   into the caller's matrix (the dense third-body part as a row update per reacting species), so
   only the concentrations, exp(g/RT) table and rates stay live in registers.
 
+The same header also carries the mechanism as data tables (T_* members: per-species thermo
+coefficients, per-reaction Arrhenius/falloff/Troe/stoichiometry records, efficiency sets and a
+species -> reaction list of net coefficients) for the cooperative kernels, where one group of lanes
+integrates one cell and each lane runs one species or reaction through a single code path.
+
 Output: pelec_amd/csrc/gen/mech_<name>.h, struct ``pelec::mech::<name>``.
 """
 from __future__ import annotations
@@ -480,6 +485,136 @@ class Gen:
             w(f"    out[{j}] = {' + '.join(terms) if terms else '0.0'};")
         w("  }")
 
+    # ------------------------------------------------------------------ data tables
+    def emit_tables(self):
+        """The mechanism as data, for the cooperative (one lane group per cell) kernels, which run
+        one code path for every species / reaction lane instead of the straight-line code above.
+        Coefficients are the same Python values the straight-line emitters print."""
+        N, R = self.N, self.R
+        w = self.w
+        sp = self.m.species
+        rx = self.m.reactions
+
+        def arr(name, ctype, vals, shape):
+            dims = "".join(f"[{d}]" for d in shape)
+
+            def fmt(v):
+                if isinstance(v, list):
+                    return "{" + ", ".join(fmt(x) for x in v) + "}"
+                return f(v) if ctype == "double" else str(int(v))
+            w(f"  static constexpr {ctype} {name}{dims} = {fmt(vals)};")
+
+        w("  // ---- data tables (cooperative kernels)")
+        w(f"  static constexpr double T_REFC = {f(M.P_ATM / M.R_ERG)};  // P_atm / R, times 1/T")
+        w(f"  static constexpr double T_IREFC = {f(M.R_ERG / M.P_ATM)};")
+        w(f"  static constexpr double T_LN10 = {f(LN10)};")
+        arr("T_TMID", "double", [s.Tmid for s in sp], [N])
+        arr("T_TMIDG", "int", [self.tmids.index(s.Tmid) for s in sp], [N])
+        gco, hco = [], []
+        for s in sp:
+            gb, hb = [], []
+            for a in (s.low, s.high):
+                gb.append([a[0] - a[6], -a[1] / 2, -a[2] / 6, -a[3] / 12, -a[4] / 20, a[0], a[5]])
+                hb.append([a[0], a[1] / 2, a[2] / 3, a[3] / 4, a[4] / 5, a[5]])
+            gco.append(gb)
+            hco.append(hb)
+        arr("T_G", "double", gco, [N, 2, 7])
+        arr("T_H", "double", hco, [N, 2, 6])
+        arr("T_E", "double", [self.ecoef[k] for k in range(N)], [N, 2, 6])
+        arr("T_CV", "double", [self.cvcoef[k] for k in range(N)], [N, 2, 5])
+        arr("T_DCV", "double", [[[c[1], 2 * c[2], 3 * c[3], 4 * c[4]] for c in self.cvcoef[k]]
+                                for k in range(N)], [N, 2, 4])
+        # efficiency sets: d = eff - 1 (third-body sums) and eff (Jacobian)
+        ns = max(1, len(self.effsets))
+        effd = [[0.0] * N for _ in range(ns)]
+        effv = [[0.0] * N for _ in range(ns)]
+        for key, idx in self.effsets.items():
+            effd[idx] = [a - 1.0 for a in key]
+            effv[idx] = list(key)
+        w(f"  static constexpr int T_NSETS = {len(self.effsets)};")
+        arr("T_EFFD", "double", effd, [ns, N])
+        arr("T_EFFV", "double", effv, [ns, N])
+        # reactions: lane order puts falloff, then three-body reactions first, so the rounds of
+        # the reaction phase after the first are uniform elementary reactions
+        kind_id = {"elementary": 0, "three-body": 1, "falloff": 2}
+        order = sorted(range(R), key=lambda j: (-kind_id[rx[j].kind], j))
+        arr("T_RORD", "int", order, [R])
+        kind, rev, kconst, A, lnA, b, EaR, low, tk, troe, eff = ([] for _ in range(11))
+        rsp, rnu, psp, pnu, dnu, nsp_, nnu = ([] for _ in range(7))
+        for r in rx:
+            if r.A <= 0.0:
+                raise ValueError(f"{r.equation}: table form needs A > 0")
+            kind.append(kind_id[r.kind])
+            rev.append(1 if r.reversible else 0)
+            kconst.append(1 if (r.Ea_R == 0.0 and r.b == 0.0) else 0)
+            A.append(r.A)
+            lnA.append(math.log(r.A))
+            b.append(r.b)
+            EaR.append(r.Ea_R)
+            if r.kind == "falloff":
+                A0, b0, E0 = r.low
+                low.append([math.log(A0) - math.log(r.A), b0 - r.b, E0 - r.Ea_R])
+            else:
+                low.append([0.0, 0.0, 0.0])
+            if r.kind == "falloff" and r.troe is not None and self.troe_const(r.troe) is not None:
+                lfc = math.log10(self.troe_const(r.troe))
+                tk.append(1)
+                troe.append([lfc, -0.4 - 0.67 * lfc, 0.75 - 1.27 * lfc, LN10 * lfc, 0.0, 0.0, 0.0, 0.0])
+            elif r.kind == "falloff" and r.troe is not None:
+                a, T3, T1 = r.troe[:3]
+                T2 = r.troe[3] if len(r.troe) > 3 else 0.0
+                tk.append(2)
+                troe.append([1.0 - a, -1.0 / T3, a, -1.0 / T1, T2, 1.0 if len(r.troe) > 3 else 0.0,
+                             -(1.0 - a) / T3, -a / T1])
+            else:
+                tk.append(0)
+                troe.append([0.0] * 8)
+            eff.append(self.effsets[tuple(r.efficiencies)] if r.efficiencies is not None else -1)
+            for side, si, ni in ((r.reactants, rsp, rnu), (r.products, psp, pnu)):
+                items = sorted(side.items())
+                if len(items) > 3 or any(v != int(v) or not 1 <= v <= 3 for _, v in items):
+                    raise ValueError(f"{r.equation}: table form needs <= 3 species, nu in 1..3")
+                si.append([k for k, _ in items] + [-1] * (3 - len(items)))
+                ni.append([int(v) for _, v in items] + [0] * (3 - len(items)))
+            d = sum(r.products.values()) - sum(r.reactants.values())
+            if d != int(d) or abs(d) > 2:
+                raise ValueError(f"{r.equation}: table form needs |dnu| <= 2")
+            dnu.append(int(d))
+            net = sorted(self.nu_of(r).items())
+            nsp_.append([k for k, _ in net] + [-1] * (6 - len(net)))
+            nnu.append([v for _, v in net] + [0.0] * (6 - len(net)))
+        arr("T_KIND", "int", kind, [R])
+        arr("T_REV", "int", rev, [R])
+        arr("T_KCONST", "int", kconst, [R])
+        arr("T_A", "double", A, [R])
+        arr("T_LNA", "double", lnA, [R])
+        arr("T_B", "double", b, [R])
+        arr("T_EAR", "double", EaR, [R])
+        arr("T_LOW", "double", low, [R, 3])
+        arr("T_TROEK", "int", tk, [R])
+        arr("T_TROE", "double", troe, [R, 8])
+        arr("T_EFF", "int", eff, [R])
+        arr("T_RSP", "int", rsp, [R, 3])
+        arr("T_RNU", "int", rnu, [R, 3])
+        arr("T_PSP", "int", psp, [R, 3])
+        arr("T_PNU", "int", pnu, [R, 3])
+        arr("T_DNU", "int", dnu, [R])
+        arr("T_NSPC", "int", nsp_, [R, 6])
+        arr("T_NNU", "double", nnu, [R, 6])
+        # species -> (reaction, net nu) in reaction order (the order wdot accumulates in)
+        rows = [[] for _ in range(N)]
+        for j, r in enumerate(rx):
+            for k, v in sorted(self.nu_of(r).items()):
+                rows[k].append((j, v))
+        ptr = [0]
+        for rr in rows:
+            ptr.append(ptr[-1] + len(rr))
+        flat = [e for rr in rows for e in rr]
+        w(f"  static constexpr int T_CSR_MAX = {max(len(rr) for rr in rows)};")
+        arr("T_CSR_PTR", "int", ptr, [N + 1])
+        arr("T_CSR_J", "int", [j for j, _ in flat], [len(flat)])
+        arr("T_CSR_NU", "double", [v for _, v in flat], [len(flat)])
+
     # ------------------------------------------------------------------ top level
     def generate(self) -> str:
         m = self.m
@@ -524,6 +659,7 @@ class Gen:
         w(f"  static constexpr const char* SPECIES[{N}] = {{{names}}};")
         self.emit_thermo()
         self.emit_rates()
+        self.emit_tables()
         w("};")
         w("}  // namespace mech")
         w("}  // namespace pelec")

@@ -93,6 +93,8 @@ _SIGS = {
     "prc_eos_rey2t": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 5),
     "prc_eos_rty2hi": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 3),
     "prc_eos_rty2wdot": (ctypes.c_int, [_i64] + [ctypes.c_void_p] * 5),
+    "prc_model_eval": (ctypes.c_int, [_i64, ctypes.c_double] + [ctypes.c_void_p] * 6
+                       + [ctypes.c_int, ctypes.c_void_p]),
 }
 
 
@@ -189,8 +191,10 @@ class Reactor:
             _check(self.lib.prc_set_param_int(self.h, key.encode(), int(value)), f"set {key}")
 
     def configure(self, rtol=None, atol=None, max_order=None, max_steps=None, form=None,
-                  lu_fp32=None, kernel_timing=None, rk64_nsubsteps=None):
-        """rk64_nsubsteps: (guess, min, max) substep counts of ReactorRK64."""
+                  lu_fp32=None, kernel_timing=None, rk64_nsubsteps=None, cooperative=None):
+        """rk64_nsubsteps: (guess, min, max) substep counts of ReactorRK64. cooperative: which of
+        the batched form's kernels run one cell per 32-lane group -- True / 3 both, 1 the Newton
+        kernel, 2 the Jacobian kernel, False / 0 neither."""
         if rtol is not None:
             self.set_param("ode.rtol", float(rtol))
         if atol is not None:
@@ -205,6 +209,9 @@ class Reactor:
             self.set_param("prc.lu_fp32", 1 if lu_fp32 else 0)
         if kernel_timing is not None:
             self.set_param("prc.kernel_timing", 1 if kernel_timing else 0)
+        if cooperative is not None:
+            self.set_param("prc.cooperative",
+                           3 if cooperative is True else int(cooperative))
         if rk64_nsubsteps is not None:
             g, lo, hi = (int(x) for x in rk64_nsubsteps)
             self.set_param("rk64.nsubsteps_guess", g)
@@ -212,8 +219,8 @@ class Reactor:
             self.set_param("rk64.nsubsteps_max", hi)
         return self
 
-    SHAPE_KEYS = ("prc.iters_per_graph", "prc.blocks_newton", "prc.blocks_jac", "prc.blocks_lu",
-                  "prc.compact_pct")
+    SHAPE_KEYS = ("prc.iters_per_graph", "prc.cooperative", "prc.blocks_newton", "prc.blocks_jac",
+                  "prc.blocks_lu", "prc.compact_pct")
 
     def get_param(self, key: str) -> int:
         v = ctypes.c_int(0)
@@ -392,6 +399,19 @@ class Eos:
         _check(self.lib.prc_eos_rty2hi(T.shape[0], _ptr(T), _ptr(hi), _stream_handle(stream)),
                "prc_eos_rty2hi")
         return hi
+
+    def model(self, t, y, rhoe0, rhoesrc=None, rYsrc=None, jacobian=True, cooperative=False,
+              stream=None):
+        """Reactor RHS (and exact Jacobian) at states y (n x ncells): (ydot, J or None)."""
+        import torch
+        nc = y.shape[1]
+        ydot = torch.empty_like(y)
+        J = torch.empty((nc, y.shape[0], y.shape[0]), dtype=y.dtype, device=y.device) \
+            if jacobian else None
+        _check(self.lib.prc_model_eval(nc, float(t), _ptr(y), _ptr(rhoe0), _ptr(rhoesrc),
+                                       _ptr(rYsrc), _ptr(ydot), _ptr(J), int(bool(cooperative)),
+                                       _stream_handle(stream)), "prc_model_eval")
+        return ydot, J
 
     def rty2wdot(self, rho, T, Y, stream=None):
         import torch

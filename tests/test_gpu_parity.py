@@ -36,13 +36,13 @@ def to_dev(a, dtype=torch.float64):
 
 
 def run_gpu(wl_name, n, form="plain", lu_fp32=False, first=0, rYsrc=None, rEsrc=None, mask=None,
-            d=None, kernel_timing=False):
+            d=None, kernel_timing=False, cooperative=False):
     W = inputs.WORKLOADS[wl_name]
     if d is None:
         d = inputs.make_inputs(wl_name, n, first)
     r = Reactor(W.mech)
     r.configure(rtol=W.rtol, atol=W.atol, form=form, lu_fp32=lu_fp32,
-                kernel_timing=kernel_timing)
+                kernel_timing=kernel_timing, cooperative=cooperative)
     rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
     FC = torch.zeros_like(T)
     diag = torch.zeros((8, n), dtype=torch.int32, device=DEV)
@@ -50,7 +50,8 @@ def run_gpu(wl_name, n, form="plain", lu_fp32=False, first=0, rYsrc=None, rEsrc=
     src = None if rYsrc is None else to_dev(rYsrc)
     esrc = None if rEsrc is None else to_dev(rEsrc)
     msk = None if mask is None else to_dev(mask, torch.int32)
-    note(f"react {wl_name} n={n} form={form} lu_fp32={lu_fp32} mask={mask is not None}")
+    note(f"react {wl_name} n={n} form={form} lu_fp32={lu_fp32} cooperative={cooperative} "
+         f"mask={mask is not None}")
     tnew = r.react_flat(rY, T, rE, W.dt, rYsrc=src, rEsrc=esrc, FC=FC, mask=msk, time=0.0)
     torch.cuda.synchronize()
     assert abs(tnew - W.dt) < 1e-20
@@ -120,17 +121,84 @@ def test_react_plain_fp64_vs_oracle(wl, n, oracles):
     np.testing.assert_array_equal(out["rE"], d["rE"])
 
 
-@pytest.mark.parametrize("lu_fp32", [False, True])
-def test_react_batched_vs_oracle(lu_fp32, oracles):
+@pytest.mark.parametrize("lu_fp32,cooperative", [(False, False), (True, False), (False, True),
+                                                   (True, True)])
+def test_react_batched_vs_oracle(lu_fp32, cooperative, oracles):
     wl, n = "drm19_64", 4096
     W = inputs.WORKLOADS[wl]
-    d, out = run_gpu(wl, n, form="batched", lu_fp32=lu_fp32)
+    d, out = run_gpu(wl, n, form="batched", lu_fp32=lu_fp32, cooperative=cooperative)
     ref = oracles[W.mech].react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
     assert out["nfail"] == 0 and (out["diag"][0] == 0).all()
     dT, dY, medT = compare(out, ref)
-    note(f"batched lu_fp32={lu_fp32}: max|dT|={dT:.3e} K max|dY|={dY:.3e} median|dT|={medT:.3e} "
+    note(f"batched lu_fp32={lu_fp32} cooperative={cooperative}: max|dT|={dT:.3e} K "
+         f"max|dY|={dY:.3e} median|dT|={medT:.3e} "
          f"mean nni gpu/oracle={out['diag'][5].mean():.2f}/{ref['stats'][:, 4].mean():.2f}")
     assert dT <= GATE_T and dY <= GATE_Y
+    np.testing.assert_array_equal(out["FC"], out["diag"][2].astype(float))
+
+
+def test_react_batched_cooperative_lidryer(oracles):
+    """Cooperative kernels on the 9-species mechanism (one reaction round, two efficiency sets)."""
+    wl, n = "lidryer_32", 2048
+    W = inputs.WORKLOADS[wl]
+    d, out = run_gpu(wl, n, form="batched", lu_fp32=True, cooperative=True)
+    ref = oracles[W.mech].react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
+    assert out["nfail"] == 0 and (out["diag"][0] == 0).all()
+    dT, dY, medT = compare(out, ref)
+    note(f"lidryer cooperative: max|dT|={dT:.3e} K max|dY|={dY:.3e} median|dT|={medT:.3e}")
+    assert dT <= GATE_T and dY <= GATE_Y
+
+
+@pytest.mark.parametrize("mech,wl", [("LiDryer", "lidryer_32"), ("drm19", "drm19_64")])
+@pytest.mark.parametrize("cooperative", [False, True])
+def test_model_eval_vs_oracle(mech, wl, cooperative, oracles):
+    """RHS and exact Jacobian at given states (prc_model_eval) against the oracle's, for the
+    one-cell-per-lane model and the cooperative (one cell per 32-lane group, table-driven
+    kinetics) model. States: the workload's initial cells and the oracle's end states of the same
+    cells (radical pools), with and without rhoY / rho e forcing. Bounds: species rates within
+    1e-9 of the cell's largest species rate, the T rate within 1e-9 of its energy scale
+    max_k |e_k f_k| / (rho c_v), and Jacobian rows within 1e-8 of the row's largest entry."""
+    from oracle.kinetics_np import NpMech
+    W = inputs.WORKLOADS[wl]
+    om = oracles[mech]
+    npm = NpMech(M.load(mech))
+    d = inputs.make_inputs(wl, 48)
+    ref = om.react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
+    rY = np.concatenate([d["rY"], ref["rY"]], axis=1)
+    T = np.concatenate([d["T"], ref["T"]])
+    rE = np.concatenate([d["rE"], d["rE"]])
+    nc = T.shape[0]
+    y = np.vstack([rY, T[None]])
+    rng = np.random.default_rng(5)
+    rho = rY.sum(0)
+    rYsrc = rng.normal(size=rY.shape) * 1e-2 * rho
+    rEsrc = rng.normal(size=nc) * 1e-3 * np.abs(rE)
+    eos = Eos(mech)
+    worst = [0.0, 0.0, 0.0]
+    for forced in (False, True):
+        note(f"model eval {mech} cooperative={cooperative} forced={forced} n={nc}")
+        yd, J = eos.model(0.0, to_dev(y), to_dev(rE),
+                          rhoesrc=to_dev(rEsrc) if forced else None,
+                          rYsrc=to_dev(rYsrc) if forced else None, cooperative=cooperative)
+        torch.cuda.synchronize()
+        yd, J = yd.cpu().numpy(), J.cpu().numpy()
+        for c in range(nc):
+            args = (0.0, y[:, c], rE[c], rEsrc[c] if forced else 0.0,
+                    rYsrc[:, c] if forced else None)
+            f_ref, Tc = om.rhs(*args)
+            J_ref = om.jac(*args)
+            sc = np.abs(f_ref[:-1]).max()
+            ek = npm.e_mass(Tc)
+            cv = (y[:-1, c] * npm.cv_mass(Tc)).sum()
+            scT = (np.abs(ek * f_ref[:-1]).max() + (abs(rEsrc[c]) if forced else 0.0)) / cv
+            ey = np.abs(yd[:-1, c] - f_ref[:-1]).max() / sc
+            eT = abs(yd[-1, c] - f_ref[-1]) / scT
+            rows = np.abs(J_ref).max(1, keepdims=True)
+            rows[rows == 0.0] = 1.0
+            eJ = (np.abs(J[c] - J_ref) / rows).max()
+            worst = [max(worst[0], ey), max(worst[1], eT), max(worst[2], eJ)]
+            assert ey <= 1e-9 and eT <= 1e-9 and eJ <= 1e-8, (c, forced, ey, eT, eJ)
+    note(f"  worst rel: ydot species {worst[0]:.2e}, T {worst[1]:.2e}, J {worst[2]:.2e}")
 
 
 @pytest.mark.parametrize("wl,n,dt,rtol,atol", [("lidryer_32", 2048, None, None, None),
