@@ -94,14 +94,16 @@ int prc_set_param_real(prc_reactor* r, const char* key, double value);
 /* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.cooperative (bit 0:
  * the Newton kernel, bit 1: the Jacobian kernel run one cell per 32-lane group instead of one
  * cell per lane), prc.blocks_newton, prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 =
- * occupancy-derived), prc.compact_pct) */
+ * occupancy-derived), prc.compact_pct, prc.coop_tail_slots (once at most this many slots are in
+ * use, the Newton kernel runs one cell per 32-lane group; 0 = never) and prc.blocks_tail (its
+ * blocks per CU)) */
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
 /* Read back an integer knob (the prc.* launch-shape knobs report the values in use). */
 int prc_get_param_int(prc_reactor* r, const char* key, int* value);
 /* Launch-shape autotuner of the batched form (prc.form 1, ReactorCvode): times react calls on the
  * given sample of cells (device arrays, component-major; copied, not modified) over candidate
- * shapes -- the kernel forms (prc.cooperative), iterations per captured graph
- * (prc.iters_per_graph), blocks per CU of the Newton,
+ * shapes -- the kernel forms (prc.cooperative, prc.coop_tail_slots), iterations per captured
+ * graph (prc.iters_per_graph), blocks per CU of the Newton,
  * Jacobian and LU kernels (prc.blocks_newton / _jac / _lu), the compaction threshold
  * (prc.compact_pct) -- by coordinate descent from the occupancy-derived defaults, keeps the
  * fastest in the reactor and returns its time (ms) in *best_ms. Synchronises the stream. No

@@ -41,9 +41,15 @@ struct Shape {
   int bpc_jac = 0;
   int bpc_lu = 0;
   int compact_pct = 50;     // compact when fewer than this percentage of the slots are alive
+  // tail: once at most tail_slots slots are in use, replay a second graph whose Newton kernel is
+  // k_newton_g (bpc_tail blocks per CU): with few live cells a cell's Newton iteration is bound by
+  // one lane's latency, which the group form divides among 32 lanes. 0: off.
+  int tail_slots = 0;
+  int bpc_tail = 0;
   bool operator==(const Shape& o) const {
     return iters == o.iters && coop == o.coop && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
-           bpc_lu == o.bpc_lu && compact_pct == o.compact_pct;
+           bpc_lu == o.bpc_lu && compact_pct == o.compact_pct && tail_slots == o.tail_slots &&
+           bpc_tail == o.bpc_tail;
   }
 };
 
@@ -54,9 +60,9 @@ struct PhaseKernels {
   PhaseKernel newton, jac;
 };
 template <typename LUT>
-PhaseKernels phase_kernels(const Shape& sh) {
+PhaseKernels phase_kernels(const Shape& sh, bool tail = false) {
   PhaseKernels k{&k_newton<LUT>, &k_jac};
-  if (sh.coop & 1) k.newton = &k_newton_g<LUT, 64>;
+  if ((sh.coop & 1) || tail) k.newton = &k_newton_g<LUT, 64>;
   if (sh.coop & 2) k.jac = &k_jac_g<64>;
   return k;
 }
@@ -93,10 +99,11 @@ struct prc_reactor {
   hipStream_t cap_stream2 = nullptr;  // second branch of each captured iteration
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  hipGraphExec_t graph_tail = nullptr;  // Shape::tail_slots: the graph with k_newton_g
   int graph_lu_fp32 = -1;
   Shape shape, graph_shape;
   int ncu = 0;
-  int occ_newton = 1, occ_jac = 1, occ_lu = 1;
+  int occ_newton = 1, occ_jac = 1, occ_lu = 1, occ_tail = 1;
   int64_t last_iterations = 0;
   // PELEC_AMD_TRACE=1: one stderr line per graph replay of the batched form (live-cell counts)
   bool trace = false;
@@ -226,19 +233,20 @@ int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
   return PRC_OK;
 }
 
+// grid of the Newton kernel (main or tail graph)
+int newton_grid(const prc_reactor* r, bool tail) {
+  if (tail) return r->ncu * (r->shape.bpc_tail > 0 ? r->shape.bpc_tail : r->occ_tail);
+  return r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
+}
+
 template <typename LUT>
-int capture_graph(prc_reactor* r) {
-  if (r->graph_exec && r->graph_lu_fp32 == r->lu_fp32 && r->graph_shape == r->shape) return PRC_OK;
-  if (r->graph_exec) {
-    (void)hipGraphExecDestroy(r->graph_exec);
-    r->graph_exec = nullptr;
-  }
+int capture_one(prc_reactor* r, bool tail, hipGraphExec_t* out) {
   hipStream_t s = r->cap_stream, s2 = r->cap_stream2;
   HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
   const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
-  const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
-  const PhaseKernels pk = phase_kernels<LUT>(r->shape);
+  const int gn = newton_grid(r, tail);
+  const PhaseKernels pk = phase_kernels<LUT>(r->shape, tail);
   bool ok = true;
   for (int it = 0; it < r->shape.iters && ok; ++it) {
     const int par = it % NLISTS;
@@ -257,9 +265,21 @@ int capture_graph(prc_reactor* r) {
     if (graph) (void)hipGraphDestroy(graph);
     return PRC_EHIP;
   }
-  const hipError_t e = hipGraphInstantiate(&r->graph_exec, graph, nullptr, nullptr, 0);
+  const hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return PRC_EHIP;
+  return e == hipSuccess ? PRC_OK : PRC_EHIP;
+}
+
+template <typename LUT>
+int capture_graph(prc_reactor* r) {
+  if (r->graph_exec && r->graph_lu_fp32 == r->lu_fp32 && r->graph_shape == r->shape) return PRC_OK;
+  if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+  if (r->graph_tail) (void)hipGraphExecDestroy(r->graph_tail);
+  r->graph_exec = r->graph_tail = nullptr;
+  r->graph_lu_fp32 = -1;
+  int e = capture_one<LUT>(r, false, &r->graph_exec);
+  if (!e && r->shape.tail_slots > 0) e = capture_one<LUT>(r, true, &r->graph_tail);
+  if (e) return e;
   r->graph_lu_fp32 = r->lu_fp32;
   r->graph_shape = r->shape;
   return PRC_OK;
@@ -278,13 +298,15 @@ int occupancy_setup(prc_reactor* r) {
   r->occ_jac = std::max(o, 1);
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lu_kernel<LUT>(), 64, 0));
   r->occ_lu = std::max(o, 1);
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_newton_g<LUT, 64>, 64, 0));
+  r->occ_tail = std::max(o, 1);
   return PRC_OK;
 }
 
 // One graph's worth of iterations launched eagerly with the graph's fork/join structure and timing
 // events around each kernel; the elapsed times are added to r->kt_ms after the replay's sync.
 template <typename LUT>
-int eager_iterations(prc_reactor* r, hipStream_t s) {
+int eager_iterations(prc_reactor* r, hipStream_t s, bool tail) {
   constexpr int EV = 5;  // newton begin/end on s2; jac begin, jac end = lu begin, lu end on s
   const size_t need = (size_t)EV * r->shape.iters;
   while (r->kt_ev.size() < need) {
@@ -295,8 +317,8 @@ int eager_iterations(prc_reactor* r, hipStream_t s) {
   hipStream_t s2 = r->cap_stream2;
   const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
   const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
-  const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
-  const PhaseKernels pk = phase_kernels<LUT>(r->shape);
+  const int gn = newton_grid(r, tail);
+  const PhaseKernels pk = phase_kernels<LUT>(r->shape, tail);
   for (int it = 0; it < r->shape.iters; ++it) {
     const int par = it % NLISTS;
     hipEvent_t* e = r->kt_ev.data() + EV * it;
@@ -356,11 +378,12 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
   r->kt_ms[0] = r->kt_ms[1] = r->kt_ms[2] = 0.0;
   r->kt_launches = 0;
   for (;;) {
+    const bool tail = r->shape.tail_slots > 0 && nactive <= r->shape.tail_slots;
     if (r->kernel_timing) {
-      e = eager_iterations<LUT>(r, s);
+      e = eager_iterations<LUT>(r, s, tail);
       if (e) return e;
     } else {
-      HIP_OK(hipGraphLaunch(r->graph_exec, s));
+      HIP_OK(hipGraphLaunch(tail ? r->graph_tail : r->graph_exec, s));
     }
     iters += r->shape.iters;
     HIP_OK(hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
@@ -453,6 +476,7 @@ int launch_1d(K kern, int64_t nc, void* stream, Xs... xs) {
 
 void release(prc_reactor* r) {
   if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+  if (r->graph_tail) (void)hipGraphExecDestroy(r->graph_tail);
   if (r->cap_stream) (void)hipStreamDestroy(r->cap_stream);
   if (r->cap_stream2) (void)hipStreamDestroy(r->cap_stream2);
   if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
@@ -463,7 +487,7 @@ void release(prc_reactor* r) {
   if (r->h_ctr) (void)hipHostFree(r->h_ctr);
   if (r->d_args) (void)hipFree(r->d_args);
   if (r->d_boxes) (void)hipFree(r->d_boxes);
-  r->graph_exec = nullptr;
+  r->graph_exec = r->graph_tail = nullptr;
   for (hipEvent_t e : r->kt_ev) (void)hipEventDestroy(e);
   r->kt_ev.clear();
   r->cap_stream = nullptr;
@@ -606,6 +630,12 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "prc.cooperative")) {
     if (v < 0 || v > 3) return PRC_EINVAL;
     r->shape.coop = v;
+  } else if (!std::strcmp(key, "prc.coop_tail_slots")) {
+    if (v < 0) return PRC_EINVAL;
+    r->shape.tail_slots = v;
+  } else if (!std::strcmp(key, "prc.blocks_tail")) {
+    if (v < 0 || v > 64) return PRC_EINVAL;
+    r->shape.bpc_tail = v;
   } else if (!std::strcmp(key, "prc.blocks_newton")) {
     if (v < 0 || v > 64) return PRC_EINVAL;
     r->shape.bpc_newton = v;
@@ -640,6 +670,10 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
     *v = r->shape.compact_pct;
   else if (!std::strcmp(key, "prc.cooperative"))
     *v = r->shape.coop;
+  else if (!std::strcmp(key, "prc.coop_tail_slots"))
+    *v = r->shape.tail_slots;
+  else if (!std::strcmp(key, "prc.blocks_tail"))
+    *v = r->shape.bpc_tail > 0 ? r->shape.bpc_tail : r->occ_tail;
   else if (!std::strcmp(key, "prc.form"))
     *v = r->form;
   else if (!std::strcmp(key, "prc.lu_fp32"))
@@ -871,21 +905,27 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
     }
     r->shape = cur;
     if (!rc) rc = r->lu_fp32 ? occupancy_setup<float>(r) : occupancy_setup<double>(r);
-    const int on = r->occ_newton, oj = r->occ_jac, ol = r->occ_lu;
+    const int on = r->occ_newton, oj = r->occ_jac, ol = r->occ_lu, ot = r->occ_tail;
+    // tail graph thresholds relative to the sample (slots in use when the tail graph takes over)
+    const int nt = (int)std::min<int64_t>(ncells, 1 << 30);
     struct Axis {
       int Shape::*field;
-      int cand[3];
+      int cand[4];
     };
-    const Axis axes[] = {{&Shape::iters, {6, 15, 30}},
-                         {&Shape::bpc_newton, {on, 2 * on, 4 * on}},
-                         {&Shape::bpc_jac, {oj, 2 * oj, 4 * oj}},
-                         {&Shape::bpc_lu, {ol, 2 * ol, 4 * ol}},
-                         {&Shape::compact_pct, {25, 50, 75}}};
+    const Axis axes[] = {{&Shape::iters, {6, 15, 30, 30}},
+                         {&Shape::bpc_newton, {on, 2 * on, 4 * on, 4 * on}},
+                         {&Shape::bpc_jac, {oj, 2 * oj, 4 * oj, 4 * oj}},
+                         {&Shape::bpc_lu, {ol, 2 * ol, 4 * ol, 4 * ol}},
+                         {&Shape::compact_pct, {25, 50, 75, 75}},
+                         {&Shape::tail_slots, {0, nt / 64, nt / 16, nt / 4}},
+                         {&Shape::bpc_tail, {ot, 4 * ot, 8 * ot, 16 * ot}}};
     for (const Axis& ax : axes) {
       if (rc) break;
+      if (ax.field == &Shape::bpc_tail && cur.tail_slots == 0) continue;
       for (int c : ax.cand) {
         Shape trial = cur;
-        const bool blocks = ax.field != &Shape::iters && ax.field != &Shape::compact_pct;
+        const bool blocks = ax.field != &Shape::iters && ax.field != &Shape::compact_pct &&
+                            ax.field != &Shape::tail_slots;
         trial.*ax.field = blocks ? std::min(c, 64) : c;
         if (trial == cur) continue;
         r->shape = trial;
