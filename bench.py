@@ -280,7 +280,8 @@ def main():
         "parity_vs_oracle": dict(errs, sample_cells=int(idx.size)),
         "roofline": {"bound": "mfma", "compute": "fp64-valu", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": traffic, "kernel": "k_newton<float>",
+                     "traffic": traffic,
+                     "kernel": "Newton branch: k_newton<float> (k_newton_g<float> in the tail graph)",
                      "launches": int(launches), "avg_launch_ms": t_newton / launches * 1e3,
                      "flops_per_launch": W_newton / launches,
                      "kernel_ms_total": kt["kernel_ms"],

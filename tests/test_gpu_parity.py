@@ -456,6 +456,37 @@ def test_compute_temp_vs_oracle():
     assert np.array_equal(np.delete(g, 6, 0), np.delete(S, 6, 0))
 
 
+def test_tail_graph_switch_vs_oracle(oracles):
+    """The tail graph (cooperative Newton kernel once at most prc.coop_tail_slots slots are in use)
+    taken over mid-call, after compactions: 16384 cells, compaction below 75 % live, tail at 8192
+    slots. Same oracle gates; the kernel-timing (eager) replay of the same shape agrees."""
+    wl, n = "drm19_64", 16384
+    W = inputs.WORKLOADS[wl]
+    d = inputs.make_inputs(wl, n)
+    ref = oracles[W.mech].react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
+    outs = []
+    for kt in (False, True):
+        r = Reactor(W.mech)
+        r.configure(rtol=W.rtol, atol=W.atol, form="batched", lu_fp32=True, cooperative=2,
+                    kernel_timing=kt)
+        r.set_param("prc.compact_pct", 75)
+        r.set_param("prc.coop_tail_slots", 8192)
+        r.set_param("prc.blocks_tail", 16)
+        rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+        note(f"tail graph switch n={n} kernel_timing={kt}")
+        r.react_flat(rY, T, rE, W.dt)
+        torch.cuda.synchronize()
+        assert r.last_failures() == 0
+        assert r.shape()["prc.coop_tail_slots"] == 8192
+        out = {"rY": rY.cpu().numpy(), "T": T.cpu().numpy()}
+        r.close()
+        dT, dY, medT = compare(out, ref)
+        note(f"  max|dT|={dT:.3e} K max|dY|={dY:.3e}")
+        assert dT <= GATE_T and dY <= GATE_Y
+        outs.append(out)
+    np.testing.assert_array_equal(outs[0]["T"], outs[1]["T"])
+
+
 def test_autotune_shape_and_parity(oracles):
     """prc_autotune picks a launch shape by timing the batched form on a sample; the tuned shape
     is valid and the reactor still matches the oracle with it."""
@@ -469,6 +500,7 @@ def test_autotune_shape_and_parity(oracles):
     note(f"  tuned shape {shape}, {ms:.2f} ms")
     assert ms > 0.0 and shape["prc.iters_per_graph"] % 3 == 0
     assert all(shape[k] >= 1 for k in ("prc.blocks_newton", "prc.blocks_jac", "prc.blocks_lu"))
+    assert 0 <= shape["prc.cooperative"] <= 3 and shape["prc.coop_tail_slots"] >= 0
     rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
     r.react_flat(rY, T, rE, W.dt)
     torch.cuda.synchronize()
