@@ -89,7 +89,8 @@ int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
  * 100000: first step dt/guess, steps within [dt/max, dt/min]; the pelec.adaptrk_nsubsteps_* knobs
  * of Exec/Production/ChallengeProblem/challenge.inp:29-31) and the gfx950 execution form:
  * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
- * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32. */
+ * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32, each solve followed
+ * by "prc.lu_refine" (1 or 2, default 2) FP64 refinement steps. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);
 /* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.cooperative (bit 0:
  * the Newton kernel, bit 1: the Jacobian kernel run one cell per 32-lane group instead of one

@@ -60,6 +60,9 @@ struct GroupModel {
   static constexpr int NSETS = Mech::T_NSETS;
   static constexpr int NS = NSETS > 0 ? NSETS : 1;
   static constexpr int RROUNDS = (R + G - 1) / G;
+  // species -> reaction gathers load their table entries in batches of CB (RHS) / CBJ (Jacobian)
+  static constexpr int CB = 8, CBJ = 4;
+  static_assert(Mech::T_CSR_PAD >= CB && Mech::T_CSR_PAD >= CBJ, "table padding covers a batch");
   static_assert(n <= G, "one state component per lane");
   static_assert(NSETS <= G, "one efficiency set per lane");
 
@@ -302,8 +305,21 @@ struct GroupModel {
     Grp::sync();
     double wd = 0.0;
     if (sp) {
+      // in batches of CB entries: the table loads of a batch are independent, so a lane waits
+      // for one batch at a time instead of one entry at a time; same summation order
       const int e = Mech::T_CSR_PTR[k + 1];
-      for (int p = Mech::T_CSR_PTR[k]; p < e; ++p) wd += Mech::T_CSR_NU[p] * sh[L_Q + Mech::T_CSR_J[p]];
+      for (int p0 = Mech::T_CSR_PTR[k]; p0 < e; p0 += CB) {
+        int jj[CB];
+        double nu[CB];
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+          jj[u] = Mech::T_CSR_J[p0 + u];
+          nu[u] = Mech::T_CSR_NU[p0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u)
+          if (p0 + u < e) wd += nu[u] * sh[L_Q + jj[u]];
+      }
     }
     return wd;
   }
@@ -349,23 +365,31 @@ struct GroupModel {
       for (int s = 0; s < NS; ++s) tb[s] = 0.0;
       for (int m = 0; m < N; ++m) row[m] = 0.0;
       const int e = Mech::T_CSR_PTR[k + 1];
-      for (int p = Mech::T_CSR_PTR[k]; p < e; ++p) {
-        const int j = Mech::T_CSR_J[p];
-        const double nu = Mech::T_CSR_NU[p];
-        dwdT += nu * sh[L_DQT + j];
-        const double* dq = sh + L_DQ + 6 * j;
-        const int* rs = Mech::T_RSP[j];
-        const int* ps = Mech::T_PSP[j];
+      for (int p0 = Mech::T_CSR_PTR[k]; p0 < e; p0 += CBJ) {
+        int jj[CBJ], sl[CBJ][6], ef[CBJ];
+        double nu[CBJ];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          if (rs[s] >= 0) row[rs[s]] += nu * dq[s];
-          if (ps[s] >= 0) row[ps[s]] += nu * dq[3 + s];
+        for (int u = 0; u < CBJ; ++u) {
+          jj[u] = Mech::T_CSR_J[p0 + u];
+          nu[u] = Mech::T_CSR_NU[p0 + u];
+          ef[u] = Mech::T_CSR_EFF[p0 + u];
+#pragma unroll
+          for (int q = 0; q < 6; ++q) sl[u][q] = Mech::T_CSR_SLOT[p0 + u][q];
         }
-        const int es = Mech::T_EFF[j];
-        if (es >= 0) {
-          const double c = nu * sh[L_DQM + j];
 #pragma unroll
-          for (int s = 0; s < NS; ++s) tb[s] += (s == es) ? c : 0.0;
+        for (int u = 0; u < CBJ; ++u) {
+          if (p0 + u >= e) break;
+          const int j = jj[u];
+          dwdT += nu[u] * sh[L_DQT + j];
+          const double* dq = sh + L_DQ + 6 * j;
+#pragma unroll
+          for (int q = 0; q < 6; ++q)
+            if (sl[u][q] >= 0) row[sl[u][q]] += nu[u] * dq[q];
+          if (ef[u] >= 0) {
+            const double c = nu[u] * sh[L_DQM + j];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) tb[q] += (q == ef[u]) ? c : 0.0;
+          }
         }
       }
       if (NSETS > 0) {

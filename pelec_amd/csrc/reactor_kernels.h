@@ -488,7 +488,8 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
 
 // One Newton correction per Newton-phase slot; on convergence the error test, step completion,
 // order/step choice and the next attempt's prediction.
-template <typename LUT>
+// NREF: FP64 refinement steps after each FP32-LU solve (FP64 factors: none)
+template <typename LUT, int NREF>
 __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int par) {
   __shared__ double gb[NV * 64];
   const Args& a = *ap;
@@ -543,16 +544,21 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
           for (int e = 0; e < NV; ++e) x[e] = b[e];
           lu_solve_perm<NV, LUT>(lu, pm, x, gbl);
 #pragma unroll
-          for (int e = 0; e < NV; ++e) {
-            double jx = 0.0;
+          for (int it = 0; it < NREF; ++it) {
 #pragma unroll
-            for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
-            r[e] = b[e] - (x[e] - gammap * jx);
-            PELEC_MEMORY_FENCE();
+            for (int e = 0; e < NV; ++e) {
+              double jx = 0.0;
+#pragma unroll
+              for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
+              r[e] = b[e] - (x[e] - gammap * jx);
+              PELEC_MEMORY_FENCE();
+            }
+            lu_solve_perm<NV, LUT>(lu, pm, r, gbl);
+#pragma unroll
+            for (int e = 0; e < NV; ++e) x[e] += r[e];
           }
-          lu_solve_perm<NV, LUT>(lu, pm, r, gbl);
 #pragma unroll
-          for (int e = 0; e < NV; ++e) b[e] = x[e] + r[e];
+          for (int e = 0; e < NV; ++e) b[e] = x[e];
         }
       };
       const int r = newton_correct<NV>(S, v, ftemp, solve);
@@ -695,7 +701,7 @@ __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int p
 }
 
 // One Newton correction per Newton-phase slot, one cell per group (k_newton's work).
-template <typename LUT, int BS>
+template <typename LUT, int BS, int NREF>
 __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, int par) {
   constexpr int CPB = BS / GRP;
   __shared__ double lds[CPB * GModel::RHS_WORDS];
@@ -752,16 +758,20 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
           b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
         } else {
           // FP32 factors, one FP64 refinement step against the FP64 Jacobian (k_newton's)
-          const double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
-          double jx = 0.0;
+          double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
           const double* Jr = a.J + g * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
 #pragma unroll
-          for (int c = 0; c < NV; ++c) {
-            const double xc = GModel::Grp::bcast(x, c);
-            if (l < NV) jx += Jr[c] * xc;
+          for (int it = 0; it < NREF; ++it) {
+            double jx = 0.0;
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+              const double xc = GModel::Grp::bcast(x, c);
+              if (l < NV) jx += Jr[c] * xc;
+            }
+            const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
+            x += lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
           }
-          const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
-          b.x = x + lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
+          b.x = x;
         }
       };
       const int r = newton_correct<NV>(S, v, ftemp, solve);

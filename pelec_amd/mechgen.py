@@ -612,8 +612,16 @@ class Gen:
         flat = [e for rr in rows for e in rr]
         w(f"  static constexpr int T_CSR_MAX = {max(len(rr) for rr in rows)};")
         arr("T_CSR_PTR", "int", ptr, [N + 1])
-        arr("T_CSR_J", "int", [j for j, _ in flat], [len(flat)])
-        arr("T_CSR_NU", "double", [v for _, v in flat], [len(flat)])
+        # per entry also the reaction's six mass-action slot species and efficiency set, so a
+        # lane's gather loads depend only on the entry index; PAD entries past the end keep
+        # batched loads in bounds
+        pad = 8
+        w(f"  static constexpr int T_CSR_PAD = {pad};")
+        arr("T_CSR_J", "int", [j for j, _ in flat] + [0] * pad, [len(flat) + pad])
+        arr("T_CSR_NU", "double", [v for _, v in flat] + [0.0] * pad, [len(flat) + pad])
+        slots = [rsp[j] + psp[j] for j, _ in flat] + [[-1] * 6] * pad
+        arr("T_CSR_SLOT", "int", slots, [len(flat) + pad, 6])
+        arr("T_CSR_EFF", "int", [eff[j] for j, _ in flat] + [-1] * pad, [len(flat) + pad])
 
     # ------------------------------------------------------------------ top level
     def generate(self) -> str:
