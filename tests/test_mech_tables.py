@@ -1,0 +1,205 @@
+"""The generated mechanism data tables (Mech::T_* in pelec_amd/csrc/gen/mech_<name>.h), which the
+cooperative one-cell-per-group kernels evaluate, checked on the CPU against the oracle.
+
+The tables are parsed from the generated header. This test restates, in numpy, how the group
+kernels use them (Arrhenius, third-body and Troe falloff terms, 1/Kc from exp(g/RT) products,
+species -> reaction lists, folded NASA-7 thermo). The results must match the oracle's production
+rates and the numpy NASA-7 thermo. A wrong sign, a swapped side, a Troe parameter out of order or a
+wrong efficiency set in the generator fails here, with no GPU involved. The structural checks tie
+the tables to the mechanism data directly.
+"""
+import ast
+import re
+
+import numpy as np
+import pytest
+
+from pelec_amd import mechanism as M
+from pelec_amd import mechgen
+
+MECHS = ("LiDryer", "drm19")
+
+
+def parse_tables(mech):
+    path = mechgen.generate(mech)
+    txt = open(path).read()
+    out = {}
+    for m in re.finditer(r"static constexpr (double|int) (T_\w+)((?:\[\d+\])*) = ([^;]+);", txt):
+        ctype, name, dims, val = m.groups()
+        val = val.strip()
+        if val.startswith("{"):
+            arr = np.array(ast.literal_eval(val.replace("{", "[").replace("}", "]")),
+                           dtype=float if ctype == "double" else int)
+            shape = tuple(int(d) for d in re.findall(r"\[(\d+)\]", dims))
+            assert arr.shape == shape, (name, arr.shape, shape)
+            out[name] = arr
+        else:
+            out[name] = float(val) if ctype == "double" else int(val)
+    return out
+
+
+@pytest.fixture(scope="module", params=MECHS)
+def case(request):
+    mech = M.load(request.param)
+    return mech, parse_tables(request.param)
+
+
+def horner(T, c):
+    acc = c[-1]
+    for x in c[-2::-1]:
+        acc = acc * T + x
+    return acc
+
+
+def table_thermo(t, T):
+    """g/RT, h/RT, e_k, cv_k, dcv_k/dT of every species at T from the tables."""
+    N = t["T_TMID"].shape[0]
+    lnT, invT = np.log(T), 1.0 / T
+    b = np.where(T < t["T_TMID"], 0, 1)
+    g = np.empty(N)
+    h = np.empty(N)
+    e = np.empty(N)
+    cv = np.empty(N)
+    dcv = np.empty(N)
+    for k in range(N):
+        cg = t["T_G"][k, b[k]]
+        g[k] = horner(T, cg[:5]) - cg[5] * lnT + cg[6] * invT
+        ch = t["T_H"][k, b[k]]
+        h[k] = horner(T, ch[:5]) + ch[5] * invT
+        e[k] = horner(T, t["T_E"][k, b[k]])
+        cv[k] = horner(T, t["T_CV"][k, b[k]])
+        dcv[k] = horner(T, t["T_DCV"][k, b[k]])
+    return g, h, e, cv, dcv
+
+
+def table_rates(t, C, T):
+    """Rates of progress q_j and production rates wdot_k from the tables, as the group kernels
+    evaluate them (group_model.h: reaction(), kinetics())."""
+    R = t["T_KIND"].shape[0]
+    lnT, invT = np.log(T), 1.0 / T
+    g, _, _, _, _ = table_thermo(t, T)
+    eg = np.exp(g)
+    ctot = C.sum()
+    Mset = ctot + t["T_EFFD"] @ C
+    q = np.zeros(R)
+    for j in range(R):
+        kind = t["T_KIND"][j]
+        karr = t["T_A"][j] if t["T_KCONST"][j] else np.exp(
+            t["T_LNA"][j] + t["T_B"][j] * lnT - t["T_EAR"][j] * invT)
+        kf = karr
+        if kind == 1:
+            kf = karr * Mset[t["T_EFF"][j]]
+        elif kind == 2:
+            lw = t["T_LOW"][j]
+            k0r = np.exp(lw[0] + lw[1] * lnT - lw[2] * invT)
+            Pr = max(k0r * Mset[t["T_EFF"][j]], 1e-300)
+            tr = t["T_TROE"][j]
+            F = 1.0
+            if t["T_TROEK"][j] == 1:
+                xx = np.log10(Pr) + tr[1]
+                f1 = xx / (tr[2] - 0.14 * xx)
+                F = np.exp(tr[3] / (1.0 + f1 * f1))
+            elif t["T_TROEK"][j] == 2:
+                Fc = tr[0] * np.exp(tr[1] * T) + tr[2] * np.exp(tr[3] * T) + (
+                    np.exp(-tr[4] * invT) if tr[5] != 0.0 else 0.0)
+                lFc = np.log10(Fc)
+                xx = np.log10(Pr) + (-0.4 - 0.67 * lFc)
+                f1 = xx / ((0.75 - 1.27 * lFc) - 0.14 * xx)
+                F = np.exp(t["T_LN10"] * lFc / (1.0 + f1 * f1))
+            kf = karr * Pr / (1.0 + Pr) * F
+        Pf = Pb = num = den = 1.0
+        for s in range(3):
+            for _ in range(t["T_RNU"][j, s]):
+                Pf *= C[t["T_RSP"][j, s]]
+                den *= eg[t["T_RSP"][j, s]]
+            for _ in range(t["T_PNU"][j, s]):
+                Pb *= C[t["T_PSP"][j, s]]
+                num *= eg[t["T_PSP"][j, s]]
+        dnu = t["T_DNU"][j]
+        num *= (t["T_IREFC"] * T if dnu > 0 else t["T_REFC"] * invT) ** abs(dnu)
+        net = Pf - num / den * Pb if t["T_REV"][j] else Pf
+        q[j] = kf * net
+    N = t["T_CSR_PTR"].shape[0] - 1
+    wd = np.zeros(N)
+    scale = np.zeros(N)
+    for k in range(N):
+        for p in range(t["T_CSR_PTR"][k], t["T_CSR_PTR"][k + 1]):
+            wd[k] += t["T_CSR_NU"][p] * q[t["T_CSR_J"][p]]
+            scale[k] += abs(t["T_CSR_NU"][p] * q[t["T_CSR_J"][p]])
+    return q, wd, scale
+
+
+def test_tables_structure(case):
+    mech, t = case
+    N, R = mech.nsp, mech.nrxn
+    assert sorted(t["T_RORD"].tolist()) == list(range(R))
+    for j, r in enumerate(mech.reactions):
+        kind = {"elementary": 0, "three-body": 1, "falloff": 2}[r.kind]
+        assert t["T_KIND"][j] == kind
+        assert t["T_REV"][j] == (1 if r.reversible else 0)
+        rs = {int(k): int(v) for k, v in zip(t["T_RSP"][j], t["T_RNU"][j]) if k >= 0}
+        ps = {int(k): int(v) for k, v in zip(t["T_PSP"][j], t["T_PNU"][j]) if k >= 0}
+        assert rs == {k: int(v) for k, v in r.reactants.items()}
+        assert ps == {k: int(v) for k, v in r.products.items()}
+        if r.efficiencies is not None:
+            np.testing.assert_array_equal(t["T_EFFV"][t["T_EFF"][j]], r.efficiencies)
+            np.testing.assert_array_equal(t["T_EFFD"][t["T_EFF"][j]], np.array(r.efficiencies) - 1.0)
+        else:
+            assert t["T_EFF"][j] == -1
+        assert np.isclose(np.exp(t["T_LNA"][j]), r.A, rtol=1e-14)
+        assert t["T_B"][j] == r.b and t["T_EAR"][j] == r.Ea_R
+    # species -> (reaction, net nu) lists: exactly the nonzero net coefficients, in reaction order
+    nu = np.zeros((R, N))
+    for j, r in enumerate(mech.reactions):
+        for k, v in r.reactants.items():
+            nu[j, k] -= v
+        for k, v in r.products.items():
+            nu[j, k] += v
+    for k in range(N):
+        ent = [(int(t["T_CSR_J"][p]), float(t["T_CSR_NU"][p]))
+               for p in range(t["T_CSR_PTR"][k], t["T_CSR_PTR"][k + 1])]
+        assert ent == [(j, nu[j, k]) for j in range(R) if nu[j, k] != 0.0]
+        for p in range(t["T_CSR_PTR"][k], t["T_CSR_PTR"][k + 1]):
+            j = t["T_CSR_J"][p]
+            assert t["T_CSR_SLOT"][p].tolist() == t["T_RSP"][j].tolist() + t["T_PSP"][j].tolist()
+            assert t["T_CSR_EFF"][p] == t["T_EFF"][j]
+    assert t["T_CSR_MAX"] == max(t["T_CSR_PTR"][k + 1] - t["T_CSR_PTR"][k] for k in range(N))
+
+
+def test_table_thermo_against_numpy(case):
+    from oracle.kinetics_np import NpMech
+    mech, t = case
+    npm = NpMech(mech)
+    for T in (300.0, 650.0, 999.0, 1000.0, 1001.0, 1800.0, 2999.0):
+        g, h, e, cv, dcv = table_thermo(t, T)
+        Ta = np.array(T)
+        np.testing.assert_allclose(h, npm.h_RT(Ta), rtol=1e-13, atol=1e-12)
+        np.testing.assert_allclose(g, npm.h_RT(Ta) - npm.s_R(Ta), rtol=1e-12, atol=1e-10)
+        np.testing.assert_allclose(e, npm.e_mass(Ta), rtol=1e-12, atol=1e-6 * np.abs(e).max())
+        np.testing.assert_allclose(cv, npm.cv_mass(Ta), rtol=1e-12)
+        # dcv/dT against central differences of c_v on the same side of each species' T_mid
+        d = 1e-3
+        side = np.where(T < t["T_TMID"], -1.0, 1.0)
+        Tc = T + side * 2 * d
+        fd = np.array([(npm.cv_mass(np.array(Tc[k] + d))[k] - npm.cv_mass(np.array(Tc[k] - d))[k])
+                       / (2 * d) for k in range(len(cv))])
+        dcv_c = np.array([table_thermo(t, Tc[k])[4][k] for k in range(len(cv))])
+        np.testing.assert_allclose(dcv_c, fd, rtol=1e-6, atol=1e-9 * np.abs(cv).max())
+
+
+def test_table_rates_against_oracle(case, oracle_lib):
+    mech, t = case
+    om = oracle_lib.OracleMech(mech)
+    rng = np.random.default_rng(21)
+    N = mech.nsp
+    for it in range(24):
+        T = rng.uniform(400.0, 2800.0)
+        C = 10.0 ** rng.uniform(-14.0, -4.0, N)
+        if it % 4 == 0:
+            C[rng.integers(0, N, 3)] = 0.0  # absent species
+        q, wd, scale = table_rates(t, C, T)
+        q_ref = om.progress(C, T)
+        w_ref = om.wdot(C, T)
+        qs = np.abs(q_ref).max()
+        np.testing.assert_allclose(q, q_ref, rtol=1e-11, atol=1e-12 * qs)
+        assert (np.abs(wd - w_ref) <= 1e-11 * scale + 1e-300).all(), (it, T)
