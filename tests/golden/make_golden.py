@@ -5,7 +5,11 @@ the numpy restatement) is integrated with scipy's Radau IIA (order 5, finite-dif
 rtol 1e-12) over the workload's dt. These solutions pin the oracle's BDF integrator independently
 of CVODE's step sequence: a BDF run at tolerance tau must land within O(tau) of them.
 
-Run from the repo root:  python tests/golden/make_golden.py
+It also writes tests/golden/pmf_burned_<mech>.txt: the variable line and the last (burned-side)
+row of the reference's premixed-flame profiles (Exec/RegTests/PMF/*.dat, data files of the
+reference's regression tests), the state the partial-equilibrium test checks the thermo against.
+
+Run from the repo root:  python tests/golden/make_golden.py [--pmf-only]
 """
 import os
 import sys
@@ -54,8 +58,27 @@ def solve(wl_name, cells):
     }
 
 
+PMF_FILES = {
+    "drm19": "Exec/RegTests/PMF/PMF_CH4_1bar_300K_DRM_MixAvg.dat",
+    "LiDryer": "Exec/RegTests/PMF/LiDryer_H2_p1_phi0_4000tu0300.dat",
+}
+
+
+def pmf_rows(here, ref="/root/reference"):
+    for mech, rel in PMF_FILES.items():
+        lines = [ln for ln in open(os.path.join(ref, rel)).read().splitlines() if ln.strip()]
+        with open(os.path.join(here, f"pmf_burned_{mech}.txt"), "w") as fh:
+            fh.write(f"# {rel}: variable line and last row (x [cm], T [K], u [cm/s], rho [g/cm^3], "
+                     "species mole fractions)\n")
+            fh.write(lines[0].strip() + "\n")
+            fh.write(lines[-1].strip() + "\n")
+
+
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))
+    pmf_rows(here)
+    if "--pmf-only" in sys.argv:
+        sys.exit(0)
     for wl_name, cells in CASES.items():
         res = solve(wl_name, cells)
         np.savez(os.path.join(here, f"radau_{wl_name}.npz"), **res)

@@ -1,0 +1,74 @@
+"""Thermo data against the reference's premixed-flame profiles (Exec/RegTests/PMF/*.dat; the
+variable line and last row are committed as tests/golden/pmf_burned_<mech>.txt by
+tests/golden/make_golden.py).
+
+* Equation of state: the pressure recomputed from the file's (rho, T, X) with this build's molecular
+  weights and gas constant is the flame's 1 atm.
+* drm19 thermo: 1.5 cm behind the stoichiometric CH4/air flame (2188 K) the fast bimolecular
+  shuffle reactions, and the CH2(S) <-> CH2 collisional quenching, are in partial equilibrium.
+  Their reaction quotients Q = prod C^nu must equal this build's Kc = exp(-sum nu g/RT)
+  (P_atm/RT)^dnu. A 1 kcal/mol error in a species' enthalpy shifts ln(Q/Kc) by 0.23; the bound
+  is 0.01. The LiDryer profile (a lean flame at 1426 K) is not equilibrated and pins only the EOS.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pelec_amd import mechanism as M
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+R_ERG = 8.31446261815324e7
+P_ATM = 1013250.0
+
+
+def burned_state(mech_name):
+    lines = [ln for ln in open(os.path.join(HERE, f"pmf_burned_{mech_name}.txt"))
+             if not ln.startswith("#")]
+    names = [s.strip('"') for s in lines[0].split("=")[1].split()]
+    row = dict(zip(names, (float(x) for x in lines[1].split())))
+    mech = M.load(mech_name)
+    X = np.clip(np.array([row[s.name] for s in mech.species]), 0.0, None)
+    X /= X.sum()
+    return mech, row["temp"], row["rho"], X
+
+
+@pytest.mark.parametrize("mech_name", ["drm19", "LiDryer"])
+def test_pmf_pressure_from_eos(mech_name):
+    mech, T, rho, X = burned_state(mech_name)
+    Wbar = (X * np.array(mech.weights)).sum()
+    p = rho * R_ERG * T / Wbar
+    assert abs(p / P_ATM - 1.0) < 1e-5, p / P_ATM
+
+
+# the shuffle and quenching reactions in partial equilibrium at the drm19 burned state
+EQUILIBRATED = [
+    "O + H2 <=> H + OH",
+    "H + O2 <=> O + OH",
+    "OH + H2 <=> H + H2O",
+    "OH + OH <=> O + H2O",
+    "OH + CO <=> H + CO2",
+    "CH2(S) + N2 <=> CH2 + N2",
+    "CH2(S) + H2O <=> CH2 + H2O",
+]
+
+
+def test_drm19_partial_equilibrium_at_burned_state():
+    from oracle.kinetics_np import NpMech
+    mech, T, rho, X = burned_state("drm19")
+    npm = NpMech(mech)
+    p = rho * R_ERG * T / (X * np.array(mech.weights)).sum()
+    C = X * p / (R_ERG * T)
+    gRT = npm.h_RT(np.array(T)) - npm.s_R(np.array(T))
+    eqs = {r.equation: (j, r) for j, r in enumerate(mech.reactions)}
+    for eq in EQUILIBRATED:
+        j, r = eqs[eq]
+        assert r.reversible
+        nu = np.zeros(mech.nsp)
+        for k, v in r.reactants.items():
+            nu[k] -= v
+        for k, v in r.products.items():
+            nu[k] += v
+        lnKc = -(nu * gRT).sum() + nu.sum() * np.log(P_ATM / (R_ERG * T))
+        lnQ = (nu[nu != 0] * np.log(C[nu != 0])).sum()
+        assert abs(lnQ - lnKc) < 0.01, (eq, lnQ - lnKc)
