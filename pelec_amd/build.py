@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 #: mechanisms built by default
-DEFAULT_MECHS = ("LiDryer", "drm19")
+DEFAULT_MECHS = ("LiDryer", "drm19", "drm19_inert40")
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -4,7 +4,8 @@
 // Work is split by phase so a wave never executes another cell's rare branch:
 //   k_start   inputs -> state, f(t0,y0), initial step (cvHin), first attempt      (once per cell)
 //   k_jac     analytic Jacobian of the cells in the set-up list that need one       (~2 per cell)
-//   k_lu      Newton matrix I - gamma J factorised in LDS, one matrix per lane        (~10 per cell)
+//   k_lu_rows Newton matrix I - gamma J factorised with one row per lane, one 32- or 64-lane
+//             group per matrix                                                      (~10 per cell)
 //   k_newton  one Newton correction for every cell in the Newton phase: RHS, LU solve, convergence
 //             test, and -- when the correction converged -- the error test, step completion,
 //             order/step choice and the next step's prediction                      (~60 per cell)
@@ -159,56 +160,12 @@ __device__ inline void wave_count(unsigned long long* cnt, bool pred) {
   if (lane == __ffsll((long long)mask) - 1) atomicAdd(cnt, (unsigned long long)__popcll(mask));
 }
 
-// Dense LU with partial pivoting of the lane's matrix held in LDS at a[(r*n + c) * LANES] (one
-// column of the LDS array per lane: conflict-free). perm receives the row permutation (row i of
-// the factors is original row perm[i]); the diagonal of U is stored inverted.
-template <int n, typename T, int LANES>
-__device__ inline int lu_lds(T* a, int* perm) {
-#pragma unroll
-  for (int r = 0; r < n; ++r) perm[r * LANES] = r;
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    T amax = fabs(a[(k * n + k) * LANES]);
-    for (int i = k + 1; i < n; ++i) {
-      const T v = fabs(a[(i * n + k) * LANES]);
-      if (v > amax) {
-        amax = v;
-        p = i;
-      }
-    }
-    if (amax == T(0)) return k + 1;
-    if (p != k) {
-      for (int j = 0; j < n; ++j) {
-        const T t = a[(k * n + j) * LANES];
-        a[(k * n + j) * LANES] = a[(p * n + j) * LANES];
-        a[(p * n + j) * LANES] = t;
-      }
-      const int t = perm[k * LANES];
-      perm[k * LANES] = perm[p * LANES];
-      perm[p * LANES] = t;
-    }
-    const T inv = T(1) / a[(k * n + k) * LANES];
-    a[(k * n + k) * LANES] = inv;
-    T rowk[n];
-#pragma unroll
-    for (int j = 0; j < n; ++j)
-      if (j > k) rowk[j] = a[(k * n + j) * LANES];
-    for (int i = k + 1; i < n; ++i) {
-      const T l = a[(i * n + k) * LANES] * inv;
-      a[(i * n + k) * LANES] = l;
-#pragma unroll
-      for (int j = 0; j < n; ++j)
-        if (j > k) a[(i * n + j) * LANES] -= l * rowk[j];
-    }
-  }
-  return 0;
-}
-
 // Dense LU with partial pivoting of one n x n matrix held by a group of G lanes (G a power of two,
 // n <= G, groups aligned within the wave), lane r of the group holding row r in registers. Rows
 // are not moved: each lane tracks the position its row would occupy in the row-swapping algorithm
 // (pos), so the pivot of step k is the max |a_ik| over positions >= k with ties to the lower
-// position -- the same choice, and the same factors, as lu_lds. The pivot row is broadcast with
+// position -- the same choice, and the same factors, as the row-swapping algorithm (lu_factor in
+// bdf_cell.h, and the oracle's). The pivot row is broadcast with
 // cross-lane shuffles. On return lane r holds row pos of the factors (unit-lower multipliers left
 // of the diagonal, U right of it, the diagonal inverted) and perm[pos] = r. Returns 0, or k + 1
 // when column k has no nonzero pivot (every lane of the group returns the same value).
@@ -283,7 +240,7 @@ __device__ inline double lu_solve_group(const T (&row)[n], int perm_l, double b,
   return l < n ? double(x) : 0.0;
 }
 
-// x = (LU)^{-1} P b with factors from lu_lds (AoS block `lu`, inverted diagonal). b in/out (FP64),
+// x = (LU)^{-1} P b with factors from lu_rows (AoS block `lu`, inverted diagonal). b in/out (FP64),
 // arithmetic in T. `gb` is this lane's LDS scratch (stride 64) used to gather b by the permutation.
 template <int n, typename T>
 __device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b, double* gb) {

@@ -24,14 +24,16 @@ using Mech = pelec::mech::PELEC_MECH;
 #include "reactor_kernels.h"
 
 namespace {
-// Newton-matrix factorisation kernel: one row per lane of a 32-lane group when n <= 32, else one
-// matrix per lane in LDS
+// Newton-matrix factorisation kernel: one row per lane, a 32-lane group per matrix when n <= 32,
+// a whole wave when n <= 64. (Larger systems would need factors streamed through HBM; no
+// mechanism of this build has them.)
 template <typename LUT>
 constexpr auto lu_kernel() {
+  static_assert(NV <= 64, "the batched form supports n = N + 1 <= 64");
   if constexpr (NV <= 32)
-    return &k_lu_rows<LUT>;
+    return &k_lu_rows<LUT, 32>;
   else
-    return &k_lu<LUT>;
+    return &k_lu_rows<LUT, 64>;
 }
 // launch shape of the batched form (prc.* knobs; prc_autotune picks them by measurement)
 struct Shape {
@@ -66,8 +68,10 @@ struct PhaseKernels {
 template <typename LUT, int NREF>
 PhaseKernels phase_kernels_r(const Shape& sh, bool tail) {
   PhaseKernels k{&k_newton<LUT, NREF>, &k_jac};
-  if ((sh.coop & 1) || tail) k.newton = &k_newton_g<LUT, 64, NREF>;
-  if (sh.coop & 2) k.jac = &k_jac_g<64>;
+  if constexpr (COOP) {
+    if ((sh.coop & 1) || tail) k.newton = &k_newton_g<LUT, 64, NREF>;
+    if (sh.coop & 2) k.jac = &k_jac_g<64>;
+  }
   return k;
 }
 template <typename LUT>
@@ -122,6 +126,9 @@ struct prc_reactor {
   int adj_seg = 8, adj_kmax = 128, adj_tcap = 256;
   double* aws = nullptr;
   size_t aws_bytes = 0;
+  // plain form above PLAIN_PRIVATE_N: matrices of the box's cells
+  double* pws = nullptr;
+  int64_t pws_cells = 0;
   // react_state workspace: rhoY (N), T, rho e, rho e forcing per cell of the box
   double* sws = nullptr;
   int64_t sws_cap = 0;
@@ -310,7 +317,7 @@ int occupancy_setup(prc_reactor* r) {
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, lu_kernel<LUT>(), 64, 0));
   r->occ_lu = std::max(o, 1);
   HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, phase_kernels<LUT>(r->shape, true).newton,
-                                                      64, 0));
+                                                      64, 0));  // = k_newton when !COOP
   r->occ_tail = std::max(o, 1);
   return PRC_OK;
 }
@@ -443,6 +450,18 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
       HIP_OK(hipGetLastError());
     }
     if (a.ncells == 0) return PRC_OK;
+    a.pws = nullptr;
+    if (PLAIN_EXT && r->integrator == 0) {
+      if (a.box.ncells > r->pws_cells) {
+        if (r->pws) (void)hipFree(r->pws);
+        r->pws = nullptr;
+        r->pws_cells = 0;
+        if (hipMalloc(&r->pws, sizeof(double) * 2 * NV * NV * a.box.ncells) != hipSuccess)
+          return PRC_ENOMEM;
+        r->pws_cells = a.box.ncells;
+      }
+      a.pws = r->pws;
+    }
     HIP_OK(hipMemcpyAsync(r->d_args, &a, sizeof(Args), hipMemcpyHostToDevice, s));
     const int64_t nb = (a.box.ncells + 63) / 64;
     if (r->integrator == 1)
@@ -520,6 +539,9 @@ void release(prc_reactor* r) {
   if (r->aws) (void)hipFree(r->aws);
   r->aws = nullptr;
   r->aws_bytes = 0;
+  if (r->pws) (void)hipFree(r->pws);
+  r->pws = nullptr;
+  r->pws_cells = 0;
   r->graph_lu_fp32 = -1;
   r->ncu = 0;
   r->initialised = false;
@@ -643,10 +665,10 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
     if (v < 1 || v > 2) return PRC_EINVAL;
     r->shape.refine = v;
   } else if (!std::strcmp(key, "prc.cooperative")) {
-    if (v < 0 || v > 3) return PRC_EINVAL;
+    if (v < 0 || v > 3 || (v && !COOP)) return PRC_EINVAL;
     r->shape.coop = v;
   } else if (!std::strcmp(key, "prc.coop_tail_slots")) {
-    if (v < 0) return PRC_EINVAL;
+    if (v < 0 || (v && !COOP)) return PRC_EINVAL;
     r->shape.tail_slots = v;
   } else if (!std::strcmp(key, "prc.blocks_tail")) {
     if (v < 0 || v > 64) return PRC_EINVAL;
@@ -907,7 +929,7 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
     if (!rc) rc = measure(best);
     Shape cur = r->shape;
     // kernel pair first (each one cell per lane or per group), at occupancy-derived block counts
-    for (int c = 0; c <= 3 && !rc; ++c) {
+    for (int c = 0; c <= (COOP ? 3 : 0) && !rc; ++c) {
       Shape trial = cur;
       trial.coop = c;
       trial.bpc_newton = trial.bpc_jac = trial.bpc_lu = 0;
@@ -939,6 +961,7 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
     for (const Axis& ax : axes) {
       if (rc) break;
       if (ax.field == &Shape::bpc_tail && cur.tail_slots == 0) continue;
+      if (ax.field == &Shape::tail_slots && !COOP) continue;
       for (int c : ax.cand) {
         Shape trial = cur;
         const bool blocks = ax.field != &Shape::iters && ax.field != &Shape::compact_pct &&
@@ -1155,10 +1178,13 @@ int prc_model_eval(int64_t nc, double t, const double* y, const double* rhoe0,
   if (nc < 0 || (nc > 0 && (!y || !rhoe0 || !ydot))) return PRC_EINVAL;
   if (nc == 0) return PRC_OK;
   hipStream_t s = (hipStream_t)stream;
-  if (cooperative)
-    hipLaunchKernelGGL(k_model_group, dim3((unsigned)((nc + 1) / 2)), dim3(64), 0, s, nc, t, y,
-                       rhoe0, rhoesrc, rYsrc, ydot, J);
-  else
+  if (cooperative) {
+    if constexpr (COOP)
+      hipLaunchKernelGGL(k_model_group<64>, dim3((unsigned)((nc + 1) / 2)), dim3(64), 0, s, nc, t,
+                         y, rhoe0, rhoesrc, rYsrc, ydot, J);
+    else
+      return PRC_EINVAL;  // this mechanism has no cooperative kernels
+  } else
     hipLaunchKernelGGL(k_model_lane, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, s, nc, t, y,
                        rhoe0, rhoesrc, rYsrc, ydot, J);
   HIP_OK(hipGetLastError());

@@ -71,6 +71,9 @@ struct Args {
   int32_t* list[NLISTS];
   int32_t* remap;
   int has_src;
+  // plain form above PLAIN_PRIVATE_N: per cell of the box, its saved Jacobian and Newton matrix
+  // (2 n^2 doubles), which would not fit the private segment
+  double* pws;
 };
 
 __device__ inline void cell_ijk(const BoxDev& b, int64_t c, int& i, int& j, int& k) {
@@ -151,6 +154,12 @@ __device__ inline void store_cell(const Args& a, int64_t g, int status, const Co
 }
 
 // ------------------------------------------------------------------ plain FP64 drop-in
+// Up to this ODE size the plain kernel keeps a cell's n x n matrices in private memory; above it
+// (the 53-species mechanisms: 2 n^2 doubles = 46 KB at n = 54) the runtime cannot back the
+// private segment, and they live in a device workspace instead (Args::pws).
+constexpr int PLAIN_PRIVATE_N = 32;
+constexpr bool PLAIN_EXT = NV > PLAIN_PRIVATE_N;
+
 template <typename LUT>
 __global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
   const Args& a = *ap;
@@ -165,7 +174,13 @@ __global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
     write_diag(a, g, CELL_OK, nullptr);
     return;
   }
-  BdfCell<Mech, LUT> cell;
+  BdfCell<Mech, LUT, NoTapeRec, false, false, PLAIN_EXT> cell;
+  if constexpr (PLAIN_EXT) {
+    double* w = a.pws + c * (int64_t)(2 * NV * NV);
+    cell.savedJ = w;
+    cell.M = reinterpret_cast<LUT*>(w + NV * NV);
+    cell.Jx = nullptr;
+  }
   double y0[NV];
   double rho = 0.0;
 #pragma unroll
@@ -354,82 +369,22 @@ __global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par
   }
 }
 
-// Newton matrices of the set-up list, factorised in LDS (one matrix per lane; FP64 factors use
-// half the lanes per pass so the LDS tile stays at 32 matrices)
-template <typename LUT>
-__global__ void __launch_bounds__(64) k_lu(const Args* __restrict__ ap, int par) {
-  constexpr int LANES = sizeof(LUT) == 4 ? 64 : 32;
-  __shared__ LUT lds[NV * NV * LANES];
-  __shared__ int plds[NV * LANES];
-  const Args& a = *ap;
-  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
-  const StateBuf& st = a.st[a.ctr[CT_CUR]];
-  const int lane = threadIdx.x;
-  for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
-    const int64_t idx = base + lane;
-    const bool valid = idx < cnt;
-    const int64_t s = valid ? a.list[par][idx] : 0;
-    const int64_t g = valid ? st.cell[s] : 0;
-    BdfScalars S;
-    if (valid) load_scalars(S, st.sc, a.C, s);
-    int ier = 0;
-    for (int pass = 0; pass < 64 / LANES; ++pass) {
-      if (valid && lane / LANES == pass) {
-        const int l = lane % LANES;
-        LUT* A = lds + l;
-        int* P = plds + l;
-        const double* Jg = a.J + g * (int64_t)(NV * NV);
-        for (int e = 0; e < NV * NV; ++e) A[e * LANES] = LUT(-S.gamma * Jg[e]);
-#pragma unroll
-        for (int d = 0; d < NV; ++d) A[(d * NV + d) * LANES] += LUT(1);
-        ier = lu_lds<NV, LUT, LANES>(A, P);
-        LUT* LUg = reinterpret_cast<LUT*>(a.LU) + g * (int64_t)(NV * NV);
-        for (int e = 0; e < NV * NV; ++e) LUg[e] = A[e * LANES];
-        int32_t* pg = a.perm + g * NV;
-#pragma unroll
-        for (int d = 0; d < NV; ++d) pg[d] = P[d * LANES];
-      }
-      __syncthreads();
-    }
-    bool requeue = false;
-    if (valid) {
-      const SoAV<NV> v = soa(st, a.C, s);
-      after_setup(S);
-      if (ier != 0) {
-        const int r = conv_fail<NV>(S, v);
-        if (r >= 0) {
-          st.phase[s] = PH_DONE;
-          atomicAdd(&a.ctr[CT_FAILED], 1ull);
-          atomicAdd(&a.ctr[CT_DONE], 1ull);
-          store_cell(a, g, r, counters(S), [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
-        } else {
-          begin_attempt<NV>(S, v, a.dt);  // nflag = 1: set-up again
-          requeue = true;
-        }
-      } else {
-        st.phase[s] = PH_READY0 + par;  // Newton from the next iteration on
-      }
-      store_scalars(S, st.sc, a.C, s);
-    }
-    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], requeue, (int32_t)s);
-  }
-}
-
-// Newton matrices of the set-up list factorised by 32-lane groups, one matrix per group and one
-// row per lane in registers (lu_rows), two matrices per wave. Used when n <= 32; the per-cell
-// bookkeeping is done by the group's first lane.
-template <typename LUT>
+// Newton matrices of the set-up list factorised by G-lane groups, one matrix per group and one
+// row per lane in registers (lu_rows): two matrices per wave with G = 32 (n <= 32), one with
+// G = 64 (n <= 64, the 53-species mechanisms). The per-cell bookkeeping is done by the group's
+// first lane.
+template <typename LUT, int G>
 __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int par) {
-  constexpr int G = 32;
-  static_assert(NV <= G, "k_lu_rows needs n <= 32");
+  static_assert(NV <= G && (G == 32 || G == 64), "k_lu_rows: one row per lane of a 32/64 group");
+  constexpr int MPW = 64 / G;  // matrices per wave
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int lane = threadIdx.x;
   const int r = lane & (G - 1);
   const bool leader = r == 0;
-  for (int64_t base = (int64_t)blockIdx.x * 2; base < cnt; base += (int64_t)gridDim.x * 2) {
-    const int64_t idx = base + (lane >> 5);
+  for (int64_t base = (int64_t)blockIdx.x * MPW; base < cnt; base += (int64_t)gridDim.x * MPW) {
+    const int64_t idx = base + lane / G;
     const bool valid = idx < cnt;
     const int64_t s = valid ? a.list[par][idx] : 0;
     const int64_t g = valid ? st.cell[s] : 0;
@@ -635,8 +590,12 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
 // kernels, which they replace one for one (prc.cooperative).
 constexpr int GRP = 32;
 using GModel = GroupModel<Mech, GRP>;
+// The cooperative kernels exist for mechanisms whose data tables were generated and whose state
+// fits one lane per component (n <= 32); for others only the one-cell-per-lane kernels are built.
+constexpr bool COOP = Mech::T_OK && NV <= GRP;
 
 // store_cell for a group: lane k < N writes rhoY_k, T(e_end, Y) by the group's T(e).
+template <class GM = GModel>
 __device__ inline void store_cell_g(const Args& a, int64_t g, int status, const Counters& S,
                                     double yl, int l, double rhoe0, double rhoesrc, double* sh) {
   int64_t loc;
@@ -644,11 +603,11 @@ __device__ inline void store_cell_g(const Args& a, int64_t g, int status, const 
   int i, j, k;
   cell_ijk(b, loc, i, j, k);
   if (status == CELL_OK) {
-    const double rho = GModel::Grp::sum(l < NSP ? yl : 0.0);
+    const double rho = GM::Grp::sum(l < NSP ? yl : 0.0);
     const double irho = 1.0 / rho;
     const double rEn = rhoe0 + a.dt * rhoesrc;
-    const double Tg = GModel::Grp::bcast(yl, NSP);
-    const double Tn = GModel::T_from_e(rEn * irho, l < NSP ? yl * irho : 0.0, l, Tg, sh);
+    const double Tg = GM::Grp::bcast(yl, NSP);
+    const double Tn = GM::T_from_e(rEn * irho, l < NSP ? yl * irho : 0.0, l, Tg, sh);
     if (l < NSP) b.rY.at(i, j, k, l) = yl;
     if (l == 0) {
       b.T.at(i, j, k, 0) = Tn;
@@ -662,15 +621,15 @@ __device__ inline void store_cell_g(const Args& a, int64_t g, int status, const 
 }
 
 // Jacobians of the set-up list entries that need one, one cell per group (k_jac's work).
-template <int BS>
+template <int BS, class GM = GModel>
 __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int par) {
   constexpr int CPB = BS / GRP;
-  __shared__ double lds[CPB * GModel::JAC_WORDS];
+  __shared__ double lds[CPB * GM::JAC_WORDS];
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int l = threadIdx.x & (GRP - 1);
-  double* sh = lds + (threadIdx.x / GRP) * GModel::JAC_WORDS;
+  double* sh = lds + (threadIdx.x / GRP) * GM::JAC_WORDS;
   for (int64_t base = (int64_t)blockIdx.x * CPB; base < cnt; base += (int64_t)gridDim.x * CPB) {
     const int64_t idx = base + threadIdx.x / GRP;
     if (idx >= cnt) continue;
@@ -695,16 +654,16 @@ __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int p
       const int64_t g = st.cell[s];
       const double yl = l < NV ? st.zn[(int64_t)l * a.C + s] : 0.0;
       const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
-      GModel::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + g * (int64_t)(NV * NV));
+      GM::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + g * (int64_t)(NV * NV));
     }
   }
 }
 
 // One Newton correction per Newton-phase slot, one cell per group (k_newton's work).
-template <typename LUT, int BS, int NREF>
+template <typename LUT, int BS, int NREF, class GM = GModel>
 __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, int par) {
   constexpr int CPB = BS / GRP;
-  __shared__ double lds[CPB * GModel::RHS_WORDS];
+  __shared__ double lds[CPB * GM::RHS_WORDS];
   // the integrator scalars of each cell live in LDS, one copy per group (in registers they would
   // be replicated on every lane of the group and more than halve the occupancy)
   __shared__ BdfScalars scl[CPB];
@@ -713,7 +672,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
   const int l = threadIdx.x & (GRP - 1);
-  double* sh = lds + (threadIdx.x / GRP) * GModel::RHS_WORDS;
+  double* sh = lds + (threadIdx.x / GRP) * GM::RHS_WORDS;
   for (int64_t base = (int64_t)blockIdx.x * CPB; base < nact; base += (int64_t)gridDim.x * CPB) {
     const int64_t s = base + threadIdx.x / GRP;
     const int ph = s < nact ? st.phase[s] : PH_DONE;
@@ -735,7 +694,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
         __builtin_memcpy(&tn, &w, 8);
         const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
         const double yl = l < NV ? v.Z(0, l) + (m0 > 0 ? v.A(l) : 0.0) : 0.0;
-        ftemp.x = GModel::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
+        ftemp.x = GM::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
       }
       BdfScalars& S = scl[threadIdx.x / GRP];
       load_scalars_group<GRP>(S, st.sc, a.C, s, l);
@@ -765,7 +724,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
             double jx = 0.0;
 #pragma unroll
             for (int c = 0; c < NV; ++c) {
-              const double xc = GModel::Grp::bcast(x, c);
+              const double xc = GM::Grp::bcast(x, c);
               if (l < NV) jx += Jr[c] * xc;
             }
             const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
@@ -809,7 +768,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
               if (e == -2) {
                 const double yl = l < NV ? v.Z(0, l) : 0.0;
                 S.nfe++;
-                const double fl = GModel::rhs(S.tn, yl, l, rhoe0, rhoesrc, srcl, sh);
+                const double fl = GM::rhs(S.tn, yl, l, rhoe0, rhoesrc, srcl, sh);
                 if (l < NV) v.Z(1, l) = S.h * fl;
               }
               attempt = true;
@@ -824,7 +783,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
           st.phase[s] = PH_DONE;
           if (status != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
         }
-        store_cell_g(a, g, status, counters(S), l < NV ? v.Z(0, l) : 0.0, l, rhoe0, rhoesrc, sh);
+        store_cell_g<GM>(a, g, status, counters(S), l < NV ? v.Z(0, l) : 0.0, l, rhoe0, rhoesrc, sh);
       } else if (l == 0) {
         if (need_setup)
           st.phase[s] = PH_SETUP;
@@ -1294,20 +1253,22 @@ __global__ void __launch_bounds__(64) k_model_lane(int64_t nc, double t, const d
 }
 
 // ... or one cell per 32-lane group (GroupModel), two per block.
-__global__ void __launch_bounds__(64) k_model_group(int64_t nc, double t, const double* y,
+template <int BS, class GM = GModel>
+__global__ void __launch_bounds__(BS) k_model_group(int64_t nc, double t, const double* y,
                                                      const double* rhoe0, const double* rhoesrc,
                                                      const double* src, double* ydot, double* J) {
-  __shared__ double lds[2 * GModel::JAC_WORDS];
+  static_assert(BS == 2 * GRP, "two cells per block");
+  __shared__ double lds[2 * GM::JAC_WORDS];
   const int64_t c = (int64_t)blockIdx.x * 2 + threadIdx.x / GRP;
   const int l = threadIdx.x & (GRP - 1);
   if (c >= nc) return;
-  double* sh = lds + (threadIdx.x / GRP) * GModel::JAC_WORDS;
+  double* sh = lds + (threadIdx.x / GRP) * GM::JAC_WORDS;
   const double yl = l < NV ? y[l * nc + c] : 0.0;
   const double srcl = (src && l < NSP) ? src[l * nc + c] : 0.0;
   const double r1 = rhoesrc ? rhoesrc[c] : 0.0;
-  const double f = GModel::rhs(t, yl, l, rhoe0[c], r1, srcl, sh);
+  const double f = GM::rhs(t, yl, l, rhoe0[c], r1, srcl, sh);
   if (l < NV) ydot[l * nc + c] = f;
-  if (J) GModel::jac(t, yl, l, rhoe0[c], r1, srcl, sh, J + c * (int64_t)(NV * NV));
+  if (J) GM::jac(t, yl, l, rhoe0[c], r1, srcl, sh, J + c * (int64_t)(NV * NV));
 }
 
 }  // namespace

@@ -486,10 +486,32 @@ class Gen:
         w("  }")
 
     # ------------------------------------------------------------------ data tables
+    def table_form_issue(self):
+        """Why the data-table form cannot represent this mechanism (None when it can): the
+        cooperative kernels then are not built for it, and the one-cell-per-lane kernels run."""
+        for r in self.m.reactions:
+            if r.A <= 0.0:
+                return f"{r.equation}: A <= 0"
+            for side in (r.reactants, r.products):
+                if len(side) > 3 or any(v != int(v) or not 1 <= v <= 3 for v in side.values()):
+                    return f"{r.equation}: more than 3 species or a non-integer order on one side"
+            d = sum(r.products.values()) - sum(r.reactants.values())
+            if d != int(d) or abs(d) > 2:
+                return f"{r.equation}: |dnu| > 2"
+        if len(self.effsets) > 32:
+            return "more than 32 efficiency sets"
+        return None
+
     def emit_tables(self):
         """The mechanism as data, for the cooperative (one lane group per cell) kernels, which run
         one code path for every species / reaction lane instead of the straight-line code above.
         Coefficients are the same Python values the straight-line emitters print."""
+        issue = self.table_form_issue()
+        if issue is not None:
+            self.w(f"  // no data tables: {issue}")
+            self.w("  static constexpr bool T_OK = false;")
+            return
+        self.w("  static constexpr bool T_OK = true;")
         N, R = self.N, self.R
         w = self.w
         sp = self.m.species

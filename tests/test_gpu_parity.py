@@ -487,6 +487,44 @@ def test_tail_graph_switch_vs_oracle(oracles):
     np.testing.assert_array_equal(outs[0]["T"], outs[1]["T"])
 
 
+def test_large_system_n41_vs_oracle(oracle_lib):
+    """n = 41 > 32 (mech/drm19_inert40.yaml: drm19 plus 19 inert species with N2's thermo): the
+    code paths of the 53-species configurations -- one 64-lane group per Newton-matrix LU, the
+    one-cell-per-lane kernels, no cooperative kernels -- for plain FP64, batched FP64 and batched
+    FP32-LU, against the oracle on the same mechanism. 40 % of each cell's N2 is moved into the
+    inert species (same W and e_k, so rho and rho*e are unchanged); they must come out unchanged."""
+    m = M.load("drm19_inert40")
+    om = oracle_lib.OracleMech(m)
+    W = inputs.WORKLOADS["drm19_64"]
+    n = 1024
+    d = inputs.make_inputs("drm19_64", n)
+    iN2 = m.species_index("N2")
+    rY = np.zeros((m.nsp, n))
+    rY[:21] = d["rY"]
+    moved = 0.4 * rY[iN2]
+    rY[iN2] -= moved
+    rY[21:] = moved / 19.0
+    ref = om.react(rY, d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
+    assert (ref["status"] == 0).all()
+    for form, fp32 in (("batched", False), ("batched", True), ("plain", False)):
+        r = Reactor("drm19_inert40")
+        r.configure(rtol=W.rtol, atol=W.atol, form=form, lu_fp32=fp32)
+        if form == "batched":
+            with pytest.raises(Exception):
+                r.set_param("prc.cooperative", 2)  # no cooperative kernels above n = 32
+        rYd, T, rE = to_dev(rY), to_dev(d["T"]), to_dev(d["rE"])
+        note(f"n=41 react form={form} lu_fp32={fp32} cells={n}")
+        r.react_flat(rYd, T, rE, W.dt)
+        torch.cuda.synchronize()
+        assert r.last_failures() == 0
+        out = {"rY": rYd.cpu().numpy(), "T": T.cpu().numpy()}
+        r.close()
+        dT, dY, _ = compare(out, ref)
+        note(f"  max|dT|={dT:.3e} K max|dY|={dY:.3e}")
+        assert dT <= GATE_T and dY <= GATE_Y
+        np.testing.assert_allclose(out["rY"][21:], rY[21:], rtol=1e-12)
+
+
 def test_autotune_shape_and_parity(oracles):
     """prc_autotune picks a launch shape by timing the batched form on a sample; the tuned shape
     is valid and the reactor still matches the oracle with it."""
