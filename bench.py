@@ -141,6 +141,105 @@ def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timin
     return out
 
 
+def run_amr(args, torch, inputs, Reactor):
+    """AMR-shaped batch: every box of drm19_amr in one prc_react_boxes call of the batched
+    mixed-precision form (launch shape autotuned on a sample of its unmasked cells), against the
+    plain FP64 form called box by box; parity against the oracle on a sample of unmasked cells;
+    masked cells must come back bit-identical. value = integrated (unmasked) cells per second."""
+    from oracle.oracle_py import OracleMech
+    from pelec_amd import mechanism as M
+    b = inputs.make_amr_batch(args.amr_boxes)
+    wl = b["workload"]
+    nc = b["ncells"]
+    nact = int(b["mask"].sum())
+    N = b["rY"].shape[0]
+    log(f"amr batch: {len(b['boxes'])} boxes, {nc} cells, {nact} unmasked")
+    dev = "cuda:0"
+    d0 = {k: torch.tensor(np.ascontiguousarray(b[k]), device=dev) for k in ("rY", "T", "rE", "mask")}
+    work = {k: d0[k].clone() for k in ("rY", "T", "rE")}
+    views = []
+    for bx in b["boxes"]:
+        nx, ny, nz = bx["shape"]
+        o, n = bx["first"], nx * ny * nz
+        views.append({"lo": bx["lo"], "hi": bx["hi"],
+                      "rY": work["rY"][:, o:o + n].view(N, nz, ny, nx),
+                      "T": work["T"][o:o + n].view(1, nz, ny, nx),
+                      "rE": work["rE"][o:o + n].view(1, nz, ny, nx),
+                      "mask": d0["mask"][o:o + n].view(nz, ny, nx)})
+
+    def restore():
+        for k in work:
+            work[k].copy_(d0[k])
+
+    act = np.flatnonzero(b["mask"])
+    shape = None
+    if not args.no_tune:
+        samp = act[:: max(1, act.size // 262144)][:262144]
+        tuner = Reactor(wl.mech)
+        tuner.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=True)
+        si = torch.tensor(samp, device=dev)
+        shape, _ = tuner.autotune(d0["rY"][:, si].contiguous(), d0["T"][si].contiguous(),
+                                  d0["rE"][si].contiguous(), wl.dt)
+        tuner.close()
+        log(f"autotuned shape {shape}")
+
+    def timed(form, fp32, steps, warmup):
+        r = Reactor(wl.mech)
+        r.configure(rtol=wl.rtol, atol=wl.atol, form=form, lu_fp32=fp32)
+        if form == "batched":
+            for k, v in (shape or {}).items():
+                r.set_param(k, int(v))
+        stream = torch.cuda.current_stream()
+        times = []
+        for i in range(warmup + steps):
+            restore()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            r.react_boxes(views, wl.dt)
+            e1.record(stream)
+            e1.synchronize()
+            if i >= warmup:
+                times.append(e0.elapsed_time(e1) * 1e-3)
+                log(f"amr form={form} step {len(times)}/{steps}: {times[-1] * 1e3:.1f} ms")
+        nfail = r.last_failures()
+        r.close()
+        return {"t": float(np.median(times)), "T": work["T"].cpu().numpy(),
+                "rY": work["rY"].cpu().numpy(), "rE": work["rE"].cpu().numpy(), "nfail": nfail}
+
+    head = timed("batched", True, args.steps, args.warmup)
+    plain = timed("plain", False, 1, 0)
+    masked = np.flatnonzero(b["mask"] == 0)
+    untouched = all(np.array_equal(o[k][..., masked], b[k][..., masked])
+                    for o in (head, plain) for k in ("T", "rY", "rE"))
+    m = min(act.size, args.sample if args.sample > 0 else 8192)
+    idx = act[:: max(1, act.size // m)][:m]
+    ref = OracleMech(M.load(wl.mech)).react(b["rY"][:, idx], b["T"][idx], b["rE"][idx], wl.dt,
+                                            rtol=wl.rtol, atol=wl.atol, nthreads=host_threads())
+    rho = ref["rY"].sum(0)
+    par = {}
+    for name, o in (("batched_fp32lu", head), ("plain_fp64", plain)):
+        dT = np.abs(o["T"][idx] - ref["T"])
+        dY = (np.abs(o["rY"][:, idx] - ref["rY"]) / rho).max(0)
+        par[name] = {"max_abs_dT_K": float(dT.max()), "max_abs_dY": float(dY.max()),
+                     "cells_over_1e-3K_or_1e-8": int(((dT > 1e-3) | (dY > 1e-8)).sum()),
+                     "cells_over_1e-4K": int((dT > 1e-4).sum()),
+                     "failed_cells": int(o["nfail"])}
+    out = {"metric": "integrated cells/s, AMR-shaped batch (drm19_amr), one batched call vs "
+                     "plain FP64 box by box",
+           "value": nact / head["t"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["t"] * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": wl.name, "mechanism": wl.mech, "boxes": len(b["boxes"]),
+                      "cells": nc, "unmasked": nact, "edges": list(inputs.AMR_EDGES),
+                      "dt": wl.dt, "rtol": wl.rtol, "atol": wl.atol, "launch_shape": shape},
+           "plain_fp64": {"value": nact / plain["t"], "ms_per_step": plain["t"] * 1e3},
+           "speedup_vs_plain_fp64": plain["t"] / head["t"],
+           "masked_cells_untouched": bool(untouched),
+           "parity_vs_oracle": dict(par, sample_cells=int(idx.size))}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +253,10 @@ def main():
                     help="time one form only (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="occupancy-derived launch shape")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--amr", action="store_true",
+                    help="instead: the AMR-shaped multi-box batch (drm19_amr), batched form in one "
+                         "call against the plain form box by box")
+    ap.add_argument("--amr-boxes", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -163,6 +266,9 @@ def main():
     import torch
     from pelec_amd import flops, inputs
     from pelec_amd.reactor import Reactor
+    if args.amr:
+        run_amr(args, torch, inputs, Reactor)
+        return
 
     wl = inputs.WORKLOADS[args.workload]
     log(f"generating {wl.ncells} cells of {wl.name}")

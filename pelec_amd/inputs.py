@@ -12,6 +12,9 @@ Workloads (species order of the mechanism files):
   drm19_64:   drm19 CH4/air, 64^3 cells, T ~ U[900,2200] K, p ~ U[1,10] atm, phi ~ U[0.6,1.6];
               10 % of cells ignition-seeded: T ~ U[1400,1600] K with Y_H = Y_OH = 1e-4.
   drm19_adj:  drm19 CH4/air, 16384 cells, T ~ U[1000,1600] K, p = 1 atm, phi ~ U[0.5,2].
+  drm19_amr:  an AMR-shaped batch (make_amr_batch): boxes with edge lengths drawn from
+              {8, 16, 32, 48, 64} per axis, 20 % of the cells masked out, T ~ U[800,2400] K,
+              p ~ U[1,20] atm, phi ~ U[0.5,1.8], dt = 5e-6 s; drm19 CH4/air.
 Air is O2 + 3.76 N2 (no Ar); H2/air uses n_H2 = 2 phi, CH4/air n_CH4 = phi / 2 per O2.
 """
 from __future__ import annotations
@@ -169,3 +172,36 @@ def make_conservative_state(wl, n, seed=7, hydro=True, ufs=7):
         nonrs[1:4] = 1e-3 * S[1:4] * rng.uniform(-1, 1, (3, n)) / dt
         nonrs[4] = 1e-4 * S[4] * rng.uniform(-1, 1, n) / dt
     return S, nonrs
+
+
+AMR_EDGES = (8, 16, 32, 48, 64)
+AMR = Workload("drm19_amr", "drm19", 0, 5e-6, 1e-6, 1e-10, 5005)
+
+
+def make_amr_batch(nboxes: int = 256) -> Dict[str, object]:
+    """The drm19_amr batch: per box (lo, hi, nx, ny, nz, first cell); all cells concatenated box
+    by box, x fastest, as rY (N, ncells), T, rE and mask (int32, 0 = covered: not integrated)."""
+    wl = AMR
+    mech = M.load(wl.mech)
+    th = Thermo(mech)
+    bi = np.arange(nboxes, dtype=np.int64)
+    shapes = np.stack([np.array(AMR_EDGES)[np.minimum((uniform(wl.seed, 100 + ax, bi) * 5).astype(int), 4)]
+                       for ax in range(3)], axis=1)
+    boxes = []
+    first = 0
+    for b in range(nboxes):
+        nx, ny, nz = (int(v) for v in shapes[b])
+        lo = (0, 0, 100 * b)
+        boxes.append({"lo": lo, "hi": (nx - 1, ny - 1, lo[2] + nz - 1), "shape": (nx, ny, nz),
+                      "first": first})
+        first += nx * ny * nz
+    idx = np.arange(first, dtype=np.int64)
+    u = lambda st: uniform(wl.seed, st, idx)
+    T = 800.0 + 1600.0 * u(0)
+    p = ATM * (1.0 + 19.0 * u(1))
+    phi = 0.5 + 1.3 * u(2)
+    mask = (u(3) >= 0.2).astype(np.int32)
+    Y = _fuel_air_Y(mech, "CH4", 0.5 * phi)
+    rho, e = th.pyt2re(p, T, Y)
+    return {"boxes": boxes, "ncells": first, "rY": (rho[:, None] * Y).T.copy(), "T": T,
+            "rE": rho * e, "mask": mask, "workload": wl}
