@@ -34,8 +34,10 @@ namespace pelec {
 
 // PH_READY0 + r: Newton matrix factorised during an iteration i with i % 3 == r. The Newton kernel
 // of iteration i runs concurrently with the set-up kernels of the same iteration, so it takes
-// cells tagged by iteration i - 1 only.
-enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2, PH_READY0 = 3 };
+// cells tagged by iteration i - 1 only. PH_LUFAIL0 + r: the matrix factorised during such an
+// iteration was singular; the Newton kernel of iteration i + 1 runs the convergence-failure
+// handling for it (which needs every integrator scalar, so the factorisation kernel does not).
+enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2, PH_READY0 = 3, PH_LUFAIL0 = 6 };
 
 // Set-up lists rotate over three buffers: iteration i consumes list i%3, appends to list (i+1)%3
 // and clears list (i+2)%3 (consumed during i-1, first appended to during i+1).
@@ -120,6 +122,24 @@ __device__ inline void store_scalars(const BdfScalars& S, uint64_t* sc, int64_t 
   __builtin_memcpy(w, &S, sizeof(S));
 #pragma unroll
   for (int k = 0; k < BDF_SCALAR_WORDS; ++k) sc[k * C + s] = w[k];
+}
+
+// after_setup on the scalar record of slot s, moving only the words it reads or writes.
+__device__ inline void after_setup_slot(uint64_t* sc, int64_t C, int64_t s) {
+  constexpr int K[] = {(int)(offsetof(BdfScalars, gamma) / 8), (int)(offsetof(BdfScalars, gammap) / 8),
+                       (int)(offsetof(BdfScalars, gamrat) / 8), (int)(offsetof(BdfScalars, crate) / 8),
+                       (int)(offsetof(BdfScalars, nstlp) / 8), (int)(offsetof(BdfScalars, nst) / 8),
+                       (int)(offsetof(BdfScalars, nsetups) / 8)};
+  constexpr int NK = sizeof(K) / sizeof(K[0]);
+  uint64_t w[BDF_SCALAR_WORDS] = {};
+#pragma unroll
+  for (int q = 0; q < NK; ++q) w[K[q]] = sc[K[q] * C + s];
+  BdfScalars S;
+  __builtin_memcpy(&S, w, sizeof(S));
+  after_setup(S);
+  __builtin_memcpy(w, &S, sizeof(S));
+#pragma unroll
+  for (int q = 0; q < NK; ++q) sc[K[q] * C + s] = w[K[q]];
 }
 
 // The integrator scalars of slot s moved between structure-of-arrays storage and a copy shared by

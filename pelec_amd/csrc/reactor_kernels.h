@@ -415,29 +415,11 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
       for (int c = 0; c < NV; ++c) LUg[c] = row[c];
       a.perm[g * NV + pos] = r;
     }
-    bool requeue = false;
     if (valid && leader) {
-      BdfScalars S;
-      load_scalars(S, st.sc, a.C, s);
-      const SoAV<NV> v = soa(st, a.C, s);
-      after_setup(S);
-      if (ier != 0) {
-        const int rc = conv_fail<NV>(S, v);
-        if (rc >= 0) {
-          st.phase[s] = PH_DONE;
-          atomicAdd(&a.ctr[CT_FAILED], 1ull);
-          atomicAdd(&a.ctr[CT_DONE], 1ull);
-          store_cell(a, g, rc, counters(S), [&](int e) { return v.Z(0, e); }, st.rhoe[s], st.rhoe[a.C + s]);
-        } else {
-          begin_attempt<NV>(S, v, a.dt);  // nflag = 1: set-up again
-          requeue = true;
-        }
-      } else {
-        st.phase[s] = PH_READY0 + par;  // Newton from the next iteration on
-      }
-      store_scalars(S, st.sc, a.C, s);
+      after_setup_slot(st.sc, a.C, s);
+      // Newton from the next iteration on; the Newton kernel handles a singular matrix there
+      st.phase[s] = (ier == 0 ? PH_READY0 : PH_LUFAIL0) + par;
     }
-    wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], requeue, (int32_t)s);
   }
 }
 
@@ -460,63 +442,71 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     const int64_t s = base + lane;
     const int ph = s < nact ? st.phase[s] : PH_DONE;
     const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
+    // singular Newton matrix in the previous iteration's set-up: CVODE's convergence-failure
+    // handling, as for a Newton failure with a current Jacobian (r = 3 below)
+    const bool lufail = ph == PH_LUFAIL0 + list_prev(par);
     bool need_setup = false, done = false;
-    if (act) {
+    if (act || lufail) {
       const SoAV<NV> v = soa(st, a.C, s);
       const int64_t g = st.cell[s];
-      // f(tn, zn[0] + acor), loading only tn and m first (keeps the RHS register budget)
-      double tn;
-      {
-        const uint64_t w = st.sc[W_TN * a.C + s];
-        __builtin_memcpy(&tn, &w, 8);
-      }
-      const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
-      double ftemp[NV];
-      {
-        double y[NV];
-#pragma unroll
-        for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e) + (m0 > 0 ? v.A(e) : 0.0);
-        rhs_slot(a, st, s, tn, y, ftemp);
-      }
       BdfScalars S;
-      load_scalars(S, st.sc, a.C, s);
-      S.nfe++;
-      if (S.m == 0) {
-#pragma unroll
-        for (int e = 0; e < NV; ++e) v.A(e) = 0.0;
-      }
-      const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV);
-      const int32_t* pm = a.perm + g * NV;
-      const double* Jg = a.J + g * (int64_t)(NV * NV);
-      const double gammap = S.gammap;
-      double* gbl = gb + lane;
-      auto solve = [&](double* b) {
-        if constexpr (sizeof(LUT) == sizeof(double)) {
-          lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
-        } else {
-          double x[NV], r[NV];
-#pragma unroll
-          for (int e = 0; e < NV; ++e) x[e] = b[e];
-          lu_solve_perm<NV, LUT>(lu, pm, x, gbl);
-#pragma unroll
-          for (int it = 0; it < NREF; ++it) {
-#pragma unroll
-            for (int e = 0; e < NV; ++e) {
-              double jx = 0.0;
-#pragma unroll
-              for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
-              r[e] = b[e] - (x[e] - gammap * jx);
-              PELEC_MEMORY_FENCE();
-            }
-            lu_solve_perm<NV, LUT>(lu, pm, r, gbl);
-#pragma unroll
-            for (int e = 0; e < NV; ++e) x[e] += r[e];
-          }
-#pragma unroll
-          for (int e = 0; e < NV; ++e) b[e] = x[e];
+      int r = 3;
+      if (lufail) {
+        load_scalars(S, st.sc, a.C, s);
+      } else {
+        // f(tn, zn[0] + acor), loading only tn and m first (keeps the RHS register budget)
+        double tn;
+        {
+          const uint64_t w = st.sc[W_TN * a.C + s];
+          __builtin_memcpy(&tn, &w, 8);
         }
-      };
-      const int r = newton_correct<NV>(S, v, ftemp, solve);
+        const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
+        double ftemp[NV];
+        {
+          double y[NV];
+#pragma unroll
+          for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e) + (m0 > 0 ? v.A(e) : 0.0);
+          rhs_slot(a, st, s, tn, y, ftemp);
+        }
+        load_scalars(S, st.sc, a.C, s);
+        S.nfe++;
+        if (S.m == 0) {
+#pragma unroll
+          for (int e = 0; e < NV; ++e) v.A(e) = 0.0;
+        }
+        const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV);
+        const int32_t* pm = a.perm + g * NV;
+        const double* Jg = a.J + g * (int64_t)(NV * NV);
+        const double gammap = S.gammap;
+        double* gbl = gb + lane;
+        auto solve = [&](double* b) {
+          if constexpr (sizeof(LUT) == sizeof(double)) {
+            lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
+          } else {
+            double x[NV], res[NV];
+#pragma unroll
+            for (int e = 0; e < NV; ++e) x[e] = b[e];
+            lu_solve_perm<NV, LUT>(lu, pm, x, gbl);
+#pragma unroll
+            for (int it = 0; it < NREF; ++it) {
+#pragma unroll
+              for (int e = 0; e < NV; ++e) {
+                double jx = 0.0;
+#pragma unroll
+                for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
+                res[e] = b[e] - (x[e] - gammap * jx);
+                PELEC_MEMORY_FENCE();
+              }
+              lu_solve_perm<NV, LUT>(lu, pm, res, gbl);
+#pragma unroll
+              for (int e = 0; e < NV; ++e) x[e] += res[e];
+            }
+#pragma unroll
+            for (int e = 0; e < NV; ++e) b[e] = x[e];
+          }
+        };
+        r = newton_correct<NV>(S, v, ftemp, solve);
+      }
       int status = -1;  // <0 running, else final status
       if (r == 2) {
         S.convfail = 2;  // stale Jacobian: rebuild and restart the Newton iteration
@@ -677,63 +667,69 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
     const int64_t s = base + threadIdx.x / GRP;
     const int ph = s < nact ? st.phase[s] : PH_DONE;
     const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
+    const bool lufail = ph == PH_LUFAIL0 + list_prev(par);  // as in k_newton
     bool need_setup = false, done = false;
-    if (act) {
+    if (act || lufail) {
       const GrpV<NV, GRP> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
       const int64_t g = st.cell[s];
       const double rhoe0 = st.rhoe[s], rhoesrc = st.rhoe[a.C + s];
       const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
-      constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
-      constexpr int W_M = offsetof(BdfScalars, m) / 8;
-      constexpr int SH_M = (offsetof(BdfScalars, m) % 8) * 8;
-      LaneVal ftemp;
-      {
-        // f(tn, zn[0] + acor) before the integrator scalars are loaded (register budget)
-        double tn;
-        const uint64_t w = st.sc[W_TN * a.C + s];
-        __builtin_memcpy(&tn, &w, 8);
-        const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
-        const double yl = l < NV ? v.Z(0, l) + (m0 > 0 ? v.A(l) : 0.0) : 0.0;
-        ftemp.x = GM::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
-      }
       BdfScalars& S = scl[threadIdx.x / GRP];
-      load_scalars_group<GRP>(S, st.sc, a.C, s, l);
-      S.nfe++;
-      if (S.m == 0 && l < NV) v.A(l) = 0.0;
-      const double gammap = S.gammap;
-      auto solve = [&](LaneVal& b) {
-        LUT row[NV];
-        int pl = l;
-        if (l < NV) {
-          const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV) + l * NV;
-#pragma unroll
-          for (int c = 0; c < NV; ++c) row[c] = lr[c];
-          pl = a.perm[g * NV + l];
-        } else {
-#pragma unroll
-          for (int c = 0; c < NV; ++c) row[c] = LUT(0);
+      int r = 3;
+      if (lufail) {
+        load_scalars_group<GRP>(S, st.sc, a.C, s, l);
+      } else {
+        constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
+        constexpr int W_M = offsetof(BdfScalars, m) / 8;
+        constexpr int SH_M = (offsetof(BdfScalars, m) % 8) * 8;
+        LaneVal ftemp;
+        {
+          // f(tn, zn[0] + acor) before the integrator scalars are loaded (register budget)
+          double tn;
+          const uint64_t w = st.sc[W_TN * a.C + s];
+          __builtin_memcpy(&tn, &w, 8);
+          const int m0 = (int)(uint32_t)(st.sc[W_M * a.C + s] >> SH_M);
+          const double yl = l < NV ? v.Z(0, l) + (m0 > 0 ? v.A(l) : 0.0) : 0.0;
+          ftemp.x = GM::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
         }
-        if constexpr (sizeof(LUT) == sizeof(double)) {
-          b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
-        } else {
-          // FP32 factors, one FP64 refinement step against the FP64 Jacobian (k_newton's)
-          double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
-          const double* Jr = a.J + g * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
+        load_scalars_group<GRP>(S, st.sc, a.C, s, l);
+        S.nfe++;
+        if (S.m == 0 && l < NV) v.A(l) = 0.0;
+        const double gammap = S.gammap;
+        auto solve = [&](LaneVal& b) {
+          LUT row[NV];
+          int pl = l;
+          if (l < NV) {
+            const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV) + l * NV;
 #pragma unroll
-          for (int it = 0; it < NREF; ++it) {
-            double jx = 0.0;
+            for (int c = 0; c < NV; ++c) row[c] = lr[c];
+            pl = a.perm[g * NV + l];
+          } else {
 #pragma unroll
-            for (int c = 0; c < NV; ++c) {
-              const double xc = GM::Grp::bcast(x, c);
-              if (l < NV) jx += Jr[c] * xc;
-            }
-            const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
-            x += lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
+            for (int c = 0; c < NV; ++c) row[c] = LUT(0);
           }
-          b.x = x;
-        }
-      };
-      const int r = newton_correct<NV>(S, v, ftemp, solve);
+          if constexpr (sizeof(LUT) == sizeof(double)) {
+            b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
+          } else {
+            // FP32 factors, one FP64 refinement step against the FP64 Jacobian (k_newton's)
+            double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
+            const double* Jr = a.J + g * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
+#pragma unroll
+            for (int it = 0; it < NREF; ++it) {
+              double jx = 0.0;
+#pragma unroll
+              for (int c = 0; c < NV; ++c) {
+                const double xc = GM::Grp::bcast(x, c);
+                if (l < NV) jx += Jr[c] * xc;
+              }
+              const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
+              x += lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
+            }
+            b.x = x;
+          }
+        };
+        r = newton_correct<NV>(S, v, ftemp, solve);
+      }
       int status = -1;
       if (r == 2) {
         S.convfail = 2;
