@@ -429,6 +429,9 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
 template <typename LUT, int NREF>
 __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int par) {
   __shared__ double gb[NV * 64];
+  // each lane's integrator scalars in LDS rather than registers: k_newton runs at the register
+  // limit, and the record is touched a few times per correction
+  __shared__ BdfScalars scl[64];
   const Args& a = *ap;
   // list (par+2)%3 was consumed by the previous iteration and is first appended to by the next
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
@@ -449,7 +452,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     if (act || lufail) {
       const SoAV<NV> v = soa(st, a.C, s);
       const int64_t g = st.cell[s];
-      BdfScalars S;
+      BdfScalars& S = scl[lane];
       int r = 3;
       if (lufail) {
         load_scalars(S, st.sc, a.C, s);
