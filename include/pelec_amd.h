@@ -97,7 +97,8 @@ int prc_set_param_real(prc_reactor* r, const char* key, double value);
  * cell per lane), prc.blocks_newton, prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 =
  * occupancy-derived), prc.compact_pct, prc.coop_tail_slots (once at most this many slots are in
  * use, the Newton kernel runs one cell per 32-lane group; 0 = never) and prc.blocks_tail (its
- * blocks per CU)) */
+ * blocks per CU); for tests, prc.test_lu_fail = 1 makes the batched form's factorisation report
+ * every Newton matrix as singular, which drives CVODE's convergence-failure handling) */
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
 /* Read back an integer knob (the prc.* launch-shape knobs report the values in use). */
 int prc_get_param_int(prc_reactor* r, const char* key, int* value);

@@ -135,6 +135,9 @@ struct prc_reactor {
   // prc.kernel_timing: batched iterations launched eagerly with HIP events around every
   // k_newton / k_jac / k_lu launch (on the stream each runs on); totals of the last call
   int kernel_timing = 0;
+  // prc.test_lu_fail: the batched form's factorisation kernel reports every Newton matrix as
+  // singular, so tests can drive the convergence-failure handling from the host
+  int test_lu_fail = 0;
   double kt_ms[3] = {0.0, 0.0, 0.0};  // newton, jac, lu
   int64_t kt_launches = 0;
   std::vector<hipEvent_t> kt_ev;
@@ -487,6 +490,7 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   for (int b = 0; b < NLISTS; ++b) a.list[b] = L.list[b];
   a.remap = L.remap;
   a.has_src = has_src ? 1 : 0;
+  a.test_lu_fail = r->test_lu_fail;
   return r->lu_fp32 ? run_batched<float>(r, a, s) : run_batched<double>(r, a, s);
 }
 
@@ -687,6 +691,8 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
     r->shape.compact_pct = v;
   } else if (!std::strcmp(key, "prc.kernel_timing")) {
     r->kernel_timing = v ? 1 : 0;
+  } else if (!std::strcmp(key, "prc.test_lu_fail")) {
+    r->test_lu_fail = v ? 1 : 0;
   } else {
     return PRC_EINVAL;
   }

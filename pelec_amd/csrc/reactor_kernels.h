@@ -71,6 +71,7 @@ struct Args {
   int32_t* list[NLISTS];
   int32_t* remap;
   int has_src;
+  int test_lu_fail;  // prc.test_lu_fail: every factorisation reports a singular matrix (tests)
   // plain form above PLAIN_PRIVATE_N: per cell of the box, its saved Jacobian and Newton matrix
   // (2 n^2 doubles), which would not fit the private segment
   double* pws;
@@ -409,6 +410,7 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
     int pos = r;
     int ier = 0;
     if (valid) ier = lu_rows<NV, LUT, G>(row, r, pos);
+    if (a.test_lu_fail && ier == 0) ier = 1;
     if (valid && r < NV && ier == 0) {
       LUT* LUg = reinterpret_cast<LUT*>(a.LU) + g * (int64_t)(NV * NV) + pos * NV;
 #pragma unroll

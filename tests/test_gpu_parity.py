@@ -548,3 +548,45 @@ def test_autotune_shape_and_parity(oracles):
                                 nthreads=8)
     dT, dY, _ = compare({"rY": rY.cpu().numpy(), "T": T.cpu().numpy()}, ref)
     assert dT <= GATE_T and dY <= GATE_Y
+
+
+# ---------------------------------------------------------------- singular Newton matrices
+@pytest.mark.parametrize("coop,tail", [(0, 0), (3, 0), (0, 4096)])
+def test_singular_newton_matrix_handling(coop, tail):
+    """Every factorisation reports a singular Newton matrix (prc.test_lu_fail, set from the host).
+    The factorisation kernel tags the cell and the next iteration's Newton kernel runs CVODE's
+    convergence-failure handling (cvHandleNFlag: restore the prediction, h *= ETACF, a fresh
+    set-up) until MXNCF = 10 failures, when the cell ends with CELL_CONVFAIL and keeps its inputs.
+    Cases: the one-cell-per-lane Newton kernel, the one-cell-per-group kernel (cooperative), and
+    the tail graph's group kernel taking over from the first iteration."""
+    W = inputs.WORKLOADS["lidryer_32"]
+    n = 512
+    d = inputs.make_inputs(W.name, n)
+    r = Reactor(W.mech)
+    r.configure(rtol=W.rtol, atol=W.atol, form="batched", lu_fp32=True, cooperative=coop)
+    if tail:
+        r.set_param("prc.coop_tail_slots", tail)
+    r.set_param("prc.test_lu_fail", 1)
+    rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+    diag = torch.zeros((8, n), dtype=torch.int32, device=DEV)
+    r.set_diagnostics(diag)
+    note(f"react lidryer_32 n={n} batched, every Newton matrix reported singular, "
+         f"cooperative={coop} tail={tail}")
+    r.react_flat(rY, T, rE, W.dt)
+    torch.cuda.synchronize()
+    dg = diag.cpu().numpy()
+    assert r.last_failures() == n
+    np.testing.assert_array_equal(dg[0], 3)   # CELL_CONVFAIL
+    np.testing.assert_array_equal(dg[7], 10)  # ncfn: MXNCF convergence failures
+    np.testing.assert_array_equal(dg[1], 0)   # no step accepted
+    np.testing.assert_array_equal(dg[4], 10)  # one set-up per attempt
+    np.testing.assert_array_equal(T.cpu().numpy(), d["T"])
+    np.testing.assert_array_equal(rY.cpu().numpy(), d["rY"])
+    np.testing.assert_array_equal(rE.cpu().numpy(), d["rE"])
+    # the knob off again: the same reactor integrates normally
+    r.set_param("prc.test_lu_fail", 0)
+    T2 = to_dev(d["T"])
+    r.react_flat(to_dev(d["rY"]), T2, to_dev(d["rE"]), W.dt)
+    torch.cuda.synchronize()
+    assert r.last_failures() == 0
+    r.close()
