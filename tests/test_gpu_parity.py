@@ -590,3 +590,53 @@ def test_singular_newton_matrix_handling(coop, tail):
     torch.cuda.synchronize()
     assert r.last_failures() == 0
     r.close()
+
+
+# ---------------------------------------------------------------- AMR-shaped batch
+def test_amr_batch_vs_oracle(oracles):
+    """The AMR-shaped batch (inputs.make_amr_batch: boxes with per-axis edges from {8..64}, 20 %
+    of the cells masked out, T ~ U[800, 2400] K, p ~ U[1, 20] atm), here 8 boxes, integrated in one
+    prc_react_boxes call of the batched FP32-LU form. Masked cells come back bit-identical. On a
+    sample of unmasked cells against the oracle: cells whose step and RHS counts equal the
+    oracle's are held to the gate; a cell whose step sequence differs (a step-size decision that
+    rounding flipped, at most 1 % of the sample) must agree within 10 rtol |T|, the integrator's
+    own tolerance scale."""
+    b = inputs.make_amr_batch(8)
+    wl = b["workload"]
+    nc = b["ncells"]
+    N = b["rY"].shape[0]
+    rY, T, rE = to_dev(b["rY"]), to_dev(b["T"]), to_dev(b["rE"])
+    mask = to_dev(b["mask"], torch.int32)
+    views = []
+    for bx in b["boxes"]:
+        nx, ny, nz = bx["shape"]
+        o, n = bx["first"], nx * ny * nz
+        views.append({"lo": bx["lo"], "hi": bx["hi"], "rY": rY[:, o:o + n].view(N, nz, ny, nx),
+                      "T": T[o:o + n].view(1, nz, ny, nx), "rE": rE[o:o + n].view(1, nz, ny, nx),
+                      "mask": mask[o:o + n].view(nz, ny, nx)})
+    r = Reactor(wl.mech)
+    r.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=True)
+    diag = torch.zeros((8, nc), dtype=torch.int32, device=DEV)
+    r.set_diagnostics(diag)
+    note(f"react_boxes drm19_amr: {len(views)} boxes, {nc} cells")
+    r.react_boxes(views, wl.dt)
+    torch.cuda.synchronize()
+    assert r.last_failures() == 0
+    r.close()
+    Tn, Yn, En, dg = T.cpu().numpy(), rY.cpu().numpy(), rE.cpu().numpy(), diag.cpu().numpy()
+    off = b["mask"] == 0
+    np.testing.assert_array_equal(Tn[off], b["T"][off])
+    np.testing.assert_array_equal(Yn[:, off], b["rY"][:, off])
+    np.testing.assert_array_equal(En[off], b["rE"][off])
+    act = np.flatnonzero(~off)
+    idx = act[:: max(1, act.size // 512)][:512]
+    ref = oracles[wl.mech].react(b["rY"][:, idx], b["T"][idx], b["rE"][idx], wl.dt, rtol=wl.rtol,
+                                 atol=wl.atol)
+    dT = np.abs(Tn[idx] - ref["T"])
+    dY = (np.abs(Yn[:, idx] - ref["rY"]) / ref["rY"].sum(0)).max(0)
+    same = (dg[1, idx] == ref["stats"][:, 0]) & (dg[2, idx] == ref["stats"][:, 1])
+    note(f"sample {idx.size}: {int(same.sum())} with the oracle's step sequence, max|dT| "
+         f"{dT[same].max():.2e} K there, {dT.max():.2e} K overall")
+    assert same.mean() >= 0.99
+    assert dT[same].max() <= GATE_T and dY[same].max() <= GATE_Y
+    assert (dT <= 10 * wl.rtol * ref["T"]).all()
