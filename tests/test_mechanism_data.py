@@ -11,7 +11,7 @@ import pytest
 from pelec_amd import mechanism as M
 from pelec_amd import mechgen
 
-MECHS = ["LiDryer", "drm19"]
+MECHS = ["LiDryer", "drm19", "gri30"]
 
 PMF_ORDER = {
     # Exec/RegTests/PMF/LiDryer_H2_p1_phi0_4000tu0300.dat:1
@@ -19,7 +19,14 @@ PMF_ORDER = {
     # Exec/RegTests/PMF/PMF_CH4_1bar_300K_DRM_MixAvg.dat:1
     "drm19": "H2 H O O2 OH H2O HO2 CH2 CH2(S) CH3 CH4 CO CO2 HCO CH2O CH3O C2H4 C2H5 C2H6 N2 AR".split(),
 }
-SIZES = {"LiDryer": (9, 21), "drm19": (21, 84)}
+PMF_ORDER["gri30"] = (  # GRI-Mech 3.0's published species order (no reference file holds it)
+    "H2 H O O2 OH H2O HO2 H2O2 C CH CH2 CH2(S) CH3 CH4 CO CO2 HCO CH2O CH2OH CH3O CH3OH C2H C2H2 "
+    "C2H3 C2H4 C2H5 C2H6 HCCO CH2CO HCCOH N NH NH2 NH3 NNH NO NO2 N2O HNO CN HCN H2CN HCNN HCNO "
+    "HOCN HNCO NCO N2 AR C3H7 C3H8 CH2CHO CH3CHO").split()
+SIZES = {"LiDryer": (9, 21), "drm19": (21, 84), "gri30": (53, 325)}
+#: continuity of the published fits at T_mid, in cp/R, h/RT and s/R. GRI-Mech 3.0's C3 fits
+#: (C3H7 L 9/84, C3H8 L 4/85) jump by up to 9e-5 in cp/R; a mistyped digit breaks this by far more
+CONTINUITY = {"LiDryer": 5e-5, "drm19": 5e-5, "gri30": 1e-4}
 
 
 @pytest.mark.parametrize("name", MECHS)
@@ -27,6 +34,38 @@ def test_species_order_and_size(name):
     m = M.load(name)
     assert [s.name for s in m.species] == PMF_ORDER[name]
     assert (m.nsp, m.nrxn) == SIZES[name]
+
+
+def test_gri30_structure():
+    # GRI-Mech 3.0: 29 falloff (Troe or Lindemann) and 12 third-body reactions with efficiencies,
+    # 16 irreversible reactions, 3 duplicate pairs; drm19 is a subset of its species and shares
+    # their NASA-7 fits
+    m = M.load("gri30")
+    kinds = [r.kind for r in m.reactions]
+    assert kinds.count("falloff") == 29 and kinds.count("three-body") == 12
+    assert sum(not r.reversible for r in m.reactions) == 16
+    assert sum(r.duplicate for r in m.reactions) == 6
+    d = M.load("drm19")
+    gs = {s.name: s for s in m.species}
+    for s in d.species:
+        g = gs[s.name]
+        assert (g.low, g.high, g.Tmid, g.composition) == (s.low, s.high, s.Tmid, s.composition)
+
+
+# standard enthalpies of formation at 298.15 K, kJ/mol (JANAF tables, 4th ed.)
+DHF298 = {"H": 218.0, "O": 249.2, "N": 472.7, "C": 716.7, "H2O": -241.8, "CO": -110.5,
+          "CO2": -393.5, "CH4": -74.6, "NO": 90.3, "NO2": 33.1, "N2O": 82.05, "NH3": -45.9,
+          "C2H2": 228.2, "C2H4": 52.5, "C2H6": -84.0, "CH3OH": -201.0, "H2O2": -136.1,
+          "C3H8": -104.7, "CH3CHO": -166.2}
+
+
+def test_gri30_formation_enthalpies():
+    # h(298.15 K) of the low-range fit against tabulated heats of formation: pins the
+    # integration constants a6 that continuity alone leaves free (one shift for both ranges)
+    for s in M.load("gri30").species:
+        if s.name in DHF298:
+            h = _nasa(s.low, 298.15)[1] * 8.31446261815324e-3 * 298.15
+            assert abs(h - DHF298[s.name]) < 2.0, (s.name, h)
 
 
 @pytest.mark.parametrize("name", MECHS)
@@ -54,7 +93,7 @@ def test_thermo_continuity(name):
     # published fits are continuous at T_mid to ~1e-5; a mistyped digit breaks this by far more
     for s in M.load(name).species:
         d = _nasa(s.low, s.Tmid) - _nasa(s.high, s.Tmid)
-        assert np.abs(d).max() < 5e-5, (s.name, d)
+        assert np.abs(d).max() < CONTINUITY[name], (s.name, d)
 
 
 @pytest.mark.parametrize("name", MECHS)

@@ -24,7 +24,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 @pytest.fixture(scope="module")
 def mechs(oracle_lib):
     out = {}
-    for name in ("LiDryer", "drm19"):
+    for name in ("LiDryer", "drm19", "gri30"):
         m = M.load(name)
         out[name] = (m, NpMech(m), oracle_lib.OracleMech(m))
     return out
@@ -93,7 +93,7 @@ def test_bdf_against_radau(mechs, wl):
 
 
 # ---------------------------------------------------------------- rates and RHS
-@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+@pytest.mark.parametrize("name", ["LiDryer", "drm19", "gri30"])
 def test_rhs_matches_numpy_restatement(mechs, name):
     m, npm, om = mechs[name]
     rng = np.random.default_rng(1)
@@ -146,7 +146,7 @@ def test_troe_factor_against_numpy(oracle_lib):
             np.testing.assert_allclose(q_c[fall], q_n[fall], rtol=1e-11)
 
 
-@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+@pytest.mark.parametrize("name", ["LiDryer", "drm19", "gri30"])
 def test_conservation_of_rates(mechs, name):
     m, npm, om = mechs[name]
     rng = np.random.default_rng(2)
@@ -180,8 +180,37 @@ def test_detailed_balance_at_equilibrium(mechs):
     assert (np.abs(q) <= 1e-5 * fwd + 1e-30).all(), np.abs(q) / fwd
 
 
+def test_detailed_balance_at_equilibrium_gri30(oracle_lib):
+    # GRI-Mech 3.0 with its 16 irreversible reactions removed (they drain CH2, CH3CHO and others
+    # even at equilibrium, so the remaining network cannot balance exactly): a stoichiometric
+    # CH4/air mixture at 2400 K integrated to constant-volume equilibrium has every reaction's
+    # net rate of progress vanishing relative to its forward rate. Exercises the equilibrium
+    # constants of all 53 species through 309 reactions.
+    import copy
+    m = copy.deepcopy(M.load("gri30"))
+    m.reactions = [r for r in m.reactions if r.reversible]
+    npm, om = NpMech(m), oracle_lib.OracleMech(m)
+    N = m.nsp
+    X = np.zeros(N)
+    X[m.species_index("CH4")], X[m.species_index("O2")], X[m.species_index("N2")] = 0.5, 1.0, 3.76
+    Y = X * npm.W / (X * npm.W).sum()
+    rho, e = npm.mixture(1013250.0, 2400.0, Y)
+    out = om.react((rho * Y)[:, None], np.array([2400.0]), np.array([rho * e]), 1.0, rtol=1e-12,
+                   atol=1e-20)
+    assert out["status"][0] == 0
+    C = np.maximum(out["rY"][:, 0], 0.0) / npm.W
+    T = out["T"][0]
+    q = om.progress(C, T)
+    Tc = np.array([T])
+    kf = npm.A * np.exp(npm.b * np.log(Tc) - npm.EaR / Tc)
+    fwd = np.abs(kf * np.prod(C[None, :] ** npm.nuf, axis=-1))
+    big = fwd > 1e-12 * fwd.max()   # reactions with a measurable forward flux
+    assert big.sum() > 100
+    assert (np.abs(q[big]) <= 1e-5 * fwd[big]).all(), (np.abs(q[big]) / fwd[big]).max()
+
+
 # ---------------------------------------------------------------- thermo / EOS
-@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+@pytest.mark.parametrize("name", ["LiDryer", "drm19", "gri30"])
 def test_T_from_e_round_trip(mechs, name):
     m, npm, om = mechs[name]
     rng = np.random.default_rng(3)
@@ -266,7 +295,7 @@ def _torch_rhs(npm, y, rhoe0, rhoesrc, src, t):
     return torch.cat([f, fT[None]])
 
 
-@pytest.mark.parametrize("name", ["LiDryer", "drm19"])
+@pytest.mark.parametrize("name", ["LiDryer", "drm19", "gri30"])
 def test_jacobian_against_autograd_and_fd(mechs, name):
     m, npm, om = mechs[name]
     rng = np.random.default_rng(4)
