@@ -88,11 +88,55 @@ def time_adjoint(torch, Reactor, inputs):
 
 
 def host_threads():
+    """Threads for the CPU oracle: every core this process may run on, unless OMP_NUM_THREADS
+    sets the host share (on a one-GPU share of a larger host it does)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else n
+
+
+def host_info():
+    """The host the CPU baseline ran on: model, sockets, cores, threads, affinity, OMP_NUM_THREADS."""
+    info = {"nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket":
+                "cores_per_socket", "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception:  # lscpu absent: the counts above remain
+        pass
+    return info
+
+
+def cpu_baseline(om, wl, d, nthr, repeats=5, target_cells=32768):
+    """The FP64 CPU oracle (the same BDF algorithm, OpenMP over cells, dynamic schedule) timed on
+    a bounded sample of the workload -- every k-th cell, ~10 s of CPU work -- `repeats` times;
+    the median is reported."""
+    n = d["T"].shape[0]
+    stride = max(1, n // target_cells)
+    idx = np.arange(0, n, stride)[:target_cells]
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        om.react(d["rY"][:, idx], d["T"][idx], d["rE"][idx], wl.dt, rtol=wl.rtol, atol=wl.atol,
+                 nthreads=nthr)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": idx.size / t, "unit": "cells/s", "cores": nthr, "kind": "port",
+            "sample": f"{idx.size} cells (every {stride}th cell of {wl.name}), median of "
+                      f"{repeats} runs: {t:.2f} s (runs {', '.join(f'{x:.2f}' for x in times)} s)",
+            "host": host_info()}
 
 
 def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timing=False,
@@ -142,13 +186,16 @@ def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timin
 
 
 def run_amr(args, torch, inputs, Reactor):
-    """AMR-shaped batch: every box of drm19_amr in one prc_react_boxes call of the batched
-    mixed-precision form (launch shape autotuned on a sample of its unmasked cells), against the
-    plain FP64 form called box by box; parity against the oracle on a sample of unmasked cells;
-    masked cells must come back bit-identical. value = integrated (unmasked) cells per second."""
+    """AMR-shaped batch: every box of <mech>_amr (GRI-Mech 3.0 by default) in one prc_react_boxes
+    call of the batched mixed-precision form (launch shape autotuned on a sample of its unmasked
+    cells; the work queue refills its slots when the cells outnumber them), against the plain
+    FP64 form called box by box -- on the first --amr-plain-boxes boxes when that is set, a
+    stated subset whose per-cell rate is reported; parity against the oracle on a sample of
+    unmasked cells; masked cells must come back bit-identical. value = integrated (unmasked)
+    cells per second."""
     from oracle.oracle_py import OracleMech
     from pelec_amd import mechanism as M
-    b = inputs.make_amr_batch(args.amr_boxes)
+    b = inputs.make_amr_batch(args.amr_boxes, args.amr_mech)
     wl = b["workload"]
     nc = b["ncells"]
     nact = int(b["mask"].sum())
@@ -183,7 +230,7 @@ def run_amr(args, torch, inputs, Reactor):
         tuner.close()
         log(f"autotuned shape {shape}")
 
-    def timed(form, fp32, steps, warmup):
+    def timed(form, fp32, steps, warmup, nbox):
         r = Reactor(wl.mech)
         r.configure(rtol=wl.rtol, atol=wl.atol, form=form, lu_fp32=fp32)
         if form == "batched":
@@ -196,45 +243,60 @@ def run_amr(args, torch, inputs, Reactor):
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            r.react_boxes(views, wl.dt)
+            r.react_boxes(views[:nbox], wl.dt)
             e1.record(stream)
             e1.synchronize()
             if i >= warmup:
                 times.append(e0.elapsed_time(e1) * 1e-3)
-                log(f"amr form={form} step {len(times)}/{steps}: {times[-1] * 1e3:.1f} ms")
+                log(f"amr form={form} boxes={nbox} step {len(times)}/{steps}: "
+                    f"{times[-1] * 1e3:.1f} ms")
         nfail = r.last_failures()
+        slots = r.get_param("prc.slots") if form == "batched" else None
         r.close()
         return {"t": float(np.median(times)), "T": work["T"].cpu().numpy(),
-                "rY": work["rY"].cpu().numpy(), "rE": work["rE"].cpu().numpy(), "nfail": nfail}
+                "rY": work["rY"].cpu().numpy(), "rE": work["rE"].cpu().numpy(), "nfail": nfail,
+                "slots": slots}
 
-    head = timed("batched", True, args.steps, args.warmup)
-    plain = timed("plain", False, 1, 0)
+    nb = len(views)
+    head = timed("batched", True, args.steps, args.warmup, nb)
+    npb = nb if args.amr_plain_boxes <= 0 else min(nb, args.amr_plain_boxes)
+    plain = timed("plain", False, 1, 0, npb)
+    nc_plain = b["boxes"][npb - 1]["first"] + int(np.prod(b["boxes"][npb - 1]["shape"]))
+    nact_plain = int(b["mask"][:nc_plain].sum())
     masked = np.flatnonzero(b["mask"] == 0)
-    untouched = all(np.array_equal(o[k][..., masked], b[k][..., masked])
-                    for o in (head, plain) for k in ("T", "rY", "rE"))
+    masked_plain = masked[masked < nc_plain]
+    untouched = all(np.array_equal(head[k][..., masked], b[k][..., masked])
+                    for k in ("T", "rY", "rE")) and \
+        all(np.array_equal(plain[k][..., masked_plain], b[k][..., masked_plain])
+            for k in ("T", "rY", "rE"))
     m = min(act.size, args.sample if args.sample > 0 else 8192)
     idx = act[:: max(1, act.size // m)][:m]
     ref = OracleMech(M.load(wl.mech)).react(b["rY"][:, idx], b["T"][idx], b["rE"][idx], wl.dt,
                                             rtol=wl.rtol, atol=wl.atol, nthreads=host_threads())
     rho = ref["rY"].sum(0)
     par = {}
-    for name, o in (("batched_fp32lu", head), ("plain_fp64", plain)):
-        dT = np.abs(o["T"][idx] - ref["T"])
-        dY = (np.abs(o["rY"][:, idx] - ref["rY"]) / rho).max(0)
-        par[name] = {"max_abs_dT_K": float(dT.max()), "max_abs_dY": float(dY.max()),
+    for name, o, lim in (("batched_fp32lu", head, nc), ("plain_fp64", plain, nc_plain)):
+        sel = idx < lim
+        dT = np.abs(o["T"][idx[sel]] - ref["T"][sel])
+        dY = (np.abs(o["rY"][:, idx[sel]] - ref["rY"][:, sel]) / rho[sel]).max(0)
+        par[name] = {"cells": int(sel.sum()), "max_abs_dT_K": float(dT.max()),
+                     "max_abs_dY": float(dY.max()),
                      "cells_over_1e-3K_or_1e-8": int(((dT > 1e-3) | (dY > 1e-8)).sum()),
                      "cells_over_1e-4K": int((dT > 1e-4).sum()),
                      "failed_cells": int(o["nfail"])}
-    out = {"metric": "integrated cells/s, AMR-shaped batch (drm19_amr), one batched call vs "
+    v_plain = nact_plain / plain["t"]
+    out = {"metric": f"integrated cells/s, AMR-shaped batch ({wl.name}), one batched call vs "
                      "plain FP64 box by box",
            "value": nact / head["t"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["t"] * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": wl.name, "mechanism": wl.mech, "boxes": len(b["boxes"]),
                       "cells": nc, "unmasked": nact, "edges": list(inputs.AMR_EDGES),
-                      "dt": wl.dt, "rtol": wl.rtol, "atol": wl.atol, "launch_shape": shape},
-           "plain_fp64": {"value": nact / plain["t"], "ms_per_step": plain["t"] * 1e3},
-           "speedup_vs_plain_fp64": plain["t"] / head["t"],
+                      "dt": wl.dt, "rtol": wl.rtol, "atol": wl.atol, "launch_shape": shape,
+                      "work_queue_slots": head["slots"]},
+           "plain_fp64": {"value": v_plain, "ms_per_step": plain["t"] * 1e3, "boxes": npb,
+                          "unmasked_cells": nact_plain},
+           "speedup_vs_plain_fp64": (nact / head["t"]) / v_plain,
            "masked_cells_untouched": bool(untouched),
            "parity_vs_oracle": dict(par, sample_cells=int(idx.size))}
     print(json.dumps(out), flush=True)
@@ -247,16 +309,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="drm19_64")
     ap.add_argument("--sample", type=int, default=0,
-                    help="cells checked against the oracle (0: every cell of the workload)")
+                    help="cells checked against the oracle (0: every cell of the workload for "
+                         "LiDryer and drm19, 8192 for the 53-species mechanisms)")
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="timed calls of the plain form (0: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "plain"], default=None,
                     help="time one form only (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="occupancy-derived launch shape")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--amr", action="store_true",
-                    help="instead: the AMR-shaped multi-box batch (drm19_amr), batched form in one "
-                         "call against the plain form box by box")
+                    help="instead: the AMR-shaped multi-box batch (<mech>_amr), batched form in "
+                         "one call against the plain form box by box")
     ap.add_argument("--amr-boxes", type=int, default=256)
+    ap.add_argument("--amr-mech", default="gri30", choices=["gri30", "drm19"])
+    ap.add_argument("--amr-plain-boxes", type=int, default=0,
+                    help="time the plain form on the first K boxes only (0: all)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -291,7 +359,8 @@ def main():
         head = time_form(torch, Reactor, wl, d0, "batched", True, args.steps, args.warmup,
                          shape=shape)
     if args.only != "batched":
-        plain = time_form(torch, Reactor, wl, d0, "plain", False, args.steps, args.warmup)
+        ps = args.plain_steps if args.plain_steps > 0 else args.steps
+        plain = time_form(torch, Reactor, wl, d0, "plain", False, ps, min(args.warmup, ps))
     fp64lu = None
     if not args.only:
         # the batched form with FP64 factors (no refinement), same tuned shape: reported beside
@@ -317,7 +386,10 @@ def main():
     # reference of all forms, and, timed on the host's cores, the CPU baseline
     from oracle.oracle_py import OracleMech
     from pelec_amd import mechanism as M
-    m = n if args.sample <= 0 else min(n, args.sample)
+    # every cell of drm19 64^3 takes the oracle ~5 s on 16 threads; a 53-species mechanism's 64^3
+    # would take minutes, so it is checked on a stated stride sample
+    sample = args.sample if args.sample > 0 else (0 if wl.mech in ("LiDryer", "drm19") else 8192)
+    m = n if sample <= 0 else min(n, sample)
     stride = max(1, n // m)
     idx = np.arange(0, n, stride)[:m]
     om = OracleMech(M.load(wl.mech))
@@ -354,14 +426,16 @@ def main():
     Wp = flops.work(wl.mech, dgp[2].sum(), dgp[3].sum(), dgp[4].sum(), dgp[5].sum(), dgp[1].sum())
 
     adj = None
-    if not args.no_adjoint:
+    if not args.no_adjoint and wl.mech == "drm19":
         adj = time_adjoint(torch, Reactor, inputs)
 
     cpu = None
     if not args.no_cpu:
-        cpu = {"value": idx.size / t_oracle, "unit": "cells/s", "cores": nthr, "kind": "port",
-               "sample": f"{idx.size} cells ({'every cell' if stride == 1 else f'every {stride}th cell'}"
-                         f" of {wl.name}, ~10% ignition-seeded), {t_oracle:.1f} s wall"}
+        # ~15 s of CPU work per run, from the parity run's cost per cell
+        per_cell = t_oracle * nthr / idx.size
+        target = int(min(32768, max(256, 15.0 / per_cell)))
+        cpu = cpu_baseline(om, wl, d, nthr, target_cells=target)
+        cpu["parity_run"] = f"{idx.size} cells in {t_oracle:.1f} s on {nthr} threads"
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -388,7 +462,7 @@ def main():
         "parity_vs_oracle": dict(errs, sample_cells=int(idx.size)),
         "batched_fp64lu": {"value": n / float(np.median(fp64lu["times"])),
                            "ms_per_step": float(np.median(fp64lu["times"])) * 1e3},
-        "roofline": {"bound": "mfma", "compute": "fp64-valu", "achieved": achieved / 1e12,
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": traffic,
                      "kernel": "Newton branch: k_newton<float> (k_newton_g<float> in the tail graph)",

@@ -11,9 +11,19 @@
  * (the ReactorBase header itself lives in the absent PelePhysics submodule).
  *
  * Conventions
- *  - All array pointers are device pointers (HBM) unless stated; all calls that take a stream
- *    enqueue work on it and return without synchronising (stream-ordered like the AMReX GPU
- *    launch region the reference runs in; PeleC synchronises right after, React.cpp:230).
+ *  - All array pointers are device pointers (HBM) unless stated. Work is enqueued on the stream
+ *    passed in, after the caller's earlier work on it.
+ *  - Which calls block: prc_react_box, prc_react_flat, prc_react_boxes and prc_react_state_box
+ *    return only once the reactor's kernels on the stream have finished (react_state's unpack
+ *    kernel excepted, which is left enqueued). The plain form synchronises because its argument
+ *    block is reused by the next call; the batched form reads its work-queue counters back after
+ *    every replay of its captured graph to decide whether another is needed. PeleC synchronises
+ *    the stream right after react anyway (React.cpp:230), so the drop-in keeps the reference's
+ *    observable ordering. prc_autotune and prc_last_failures also block. prc_adjoint,
+ *    prc_compute_temp, the EOS helpers and prc_model_eval enqueue and return.
+ *  - A reactor handle owns per-call device workspaces (batched state, react_state packing,
+ *    adjoint checkpoints): calls on one handle must be issued on one stream, or ordered by the
+ *    caller, or they race on those workspaces. Separate handles are independent.
  *  - Units are CGS (g, cm, s, K, erg), as PeleC and PelePhysics use them.
  *  - Return value: PRC_OK (0) or a negative PRC_E* error code; per-cell integration failures are
  *    reported through the optional per-cell diagnostics buffer, never by aborting the batch.
@@ -90,17 +100,23 @@ int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
  * of Exec/Production/ChallengeProblem/challenge.inp:29-31) and the gfx950 execution form:
  * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
  * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32, each solve followed
- * by "prc.lu_refine" (1 or 2, default 2) FP64 refinement steps. */
+ * by "prc.lu_refine" (0, 1 or 2, default 2) FP64 refinement steps against the FP64 Jacobian. In
+ * the batched form the FP32 factors are those of D M D^-1, D the powers of two of the cell's error
+ * weights (exact scaling); the plain form factors M unscaled and always refines at least once. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);
 /* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.cooperative (bit 0:
  * the Newton kernel, bit 1: the Jacobian kernel run one cell per 32-lane group instead of one
  * cell per lane), prc.blocks_newton, prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 =
  * occupancy-derived), prc.compact_pct, prc.coop_tail_slots (once at most this many slots are in
  * use, the Newton kernel runs one cell per 32-lane group; 0 = never) and prc.blocks_tail (its
- * blocks per CU); for tests, prc.test_lu_fail = 1 makes the batched form's factorisation report
- * every Newton matrix as singular, which drives CVODE's convergence-failure handling) */
+ * blocks per CU), prc.max_slots (slots of the batched form's work queue: 0 = one per cell while
+ * half the free device memory holds them; a call with more cells than slots refills the slots
+ * that finished cells free, from the call's cell queue); for tests, prc.test_lu_fail = 1 makes
+ * the batched form's factorisation report every Newton matrix as singular, which drives CVODE's
+ * convergence-failure handling) */
 int prc_set_param_int(prc_reactor* r, const char* key, int value);
-/* Read back an integer knob (the prc.* launch-shape knobs report the values in use). */
+/* Read back an integer knob (the prc.* launch-shape knobs report the values in use; "prc.slots"
+ * gives the slot count of the batched workspace now allocated). */
 int prc_get_param_int(prc_reactor* r, const char* key, int* value);
 /* Launch-shape autotuner of the batched form (prc.form 1, ReactorCvode): times react calls on the
  * given sample of cells (device arrays, component-major; copied, not modified) over candidate
@@ -192,7 +208,9 @@ int prc_compute_temp(const int32_t lo[3], const int32_t hi[3], prc_array4 S, prc
  * recomputation ("adj.max_checkpoints" 128 per cell, "adj.segment_records" 256 tape records per
  * segment; status 5 when exceeded). status: per-cell CellStatus. tape (optional, may be NULL):
  * the complete forward tape of each cell (tape_cap records of 16 + NUM_SPECIES + 1 doubles per
- * cell; tape_count[c] = records written, -1 if tape_cap was too small) for verification. */
+ * cell; tape_count[c] = records written, -1 if tape_cap was too small) for verification.
+ * Mechanisms with NUM_SPECIES + 1 > 32 (the 53-species ones) have no adjoint kernel: its
+ * per-lane private state would exceed what the runtime backs; the call returns PRC_EINVAL. */
 int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double* T0,
                 const double* rE0, const double* rYsrc, const double* rEsrc, double dt,
                 const double* logA, const double* ct_rY, const double* ct_T, double* rY_out,

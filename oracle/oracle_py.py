@@ -13,27 +13,32 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+#: the same source built with FMA contraction (g++ -mfma -ffp-contract=fast): a second FP64
+#: evaluation of the same algorithm, whose distance from the first measures how far rounding
+#: alone moves an integration (its step decisions) on given inputs
+FMA_LIB_PATH = os.path.join(HERE, "_build", "liboracle_fma.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(FMA_LIB_PATH):
         subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
                        stdout=subprocess.DEVNULL)
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(variant: str = ""):
+    """The oracle library: variant "" (no contraction) or "fma"."""
+    if variant not in _libs:
+        path = FMA_LIB_PATH if variant == "fma" else LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.orc_mech_new.restype = ctypes.c_void_p
         L.orc_mech_new.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 4 + [_ip, _ip] + [_dp] * 8 + [_ip]
         L.orc_mech_free.argtypes = [ctypes.c_void_p]
@@ -61,8 +66,8 @@ def lib():
         L.orc_replay.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                  ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_double,
                                  _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _d(a):
@@ -79,8 +84,9 @@ STAT_NAMES = ("nst", "nfe", "nje", "nsetups", "nni", "netf", "ncfn")
 class OracleMech:
     """CPU FP64 oracle of one mechanism (tables built from pelec_amd.mechanism data)."""
 
-    def __init__(self, mech):
+    def __init__(self, mech, variant: str = ""):
         self.mech = mech
+        self.L = lib(variant)
         N, R = mech.nsp, mech.nrxn
         self.N, self.R = N, R
         self.W = np.ascontiguousarray(mech.weights, dtype=np.float64)
@@ -111,26 +117,26 @@ class OracleMech:
                 ntroe[j] = len(r.troe)
                 troe[j, : len(r.troe)] = r.troe
         self._keep = [self.W, Tmid, low, high, kind, rev, A, b, EaR, nuf, nur, eff, lowp, troe, ntroe]
-        self.h = lib().orc_mech_new(N, R, _d(self.W), _d(Tmid), _d(low), _d(high), _i(kind), _i(rev),
+        self.h = self.L.orc_mech_new(N, R, _d(self.W), _d(Tmid), _d(low), _d(high), _i(kind), _i(rev),
                                     _d(A), _d(b), _d(EaR), _d(nuf.ravel()), _d(nur.ravel()),
                                     _d(eff.ravel()), _d(lowp.ravel()), _d(troe.ravel()), _i(ntroe))
         self._keep += [nuf.ravel(), nur.ravel(), eff.ravel(), lowp.ravel(), troe.ravel()]
 
     def __del__(self):
         try:
-            lib().orc_mech_free(self.h)
+            self.L.orc_mech_free(self.h)
         except Exception:
             pass
 
     def thermo(self, T):
         cp, h, s = (np.zeros(self.N) for _ in range(3))
-        lib().orc_thermo(self.h, float(T), _d(cp), _d(h), _d(s))
+        self.L.orc_thermo(self.h, float(T), _d(cp), _d(h), _d(s))
         return cp, h, s
 
     def T_from_e(self, e, Y, Tguess):
         it = np.zeros(1, dtype=np.int32)
         Y = np.ascontiguousarray(Y, dtype=np.float64)
-        T = lib().orc_T_from_e(self.h, float(e), _d(Y), float(Tguess), _i(it))
+        T = self.L.orc_T_from_e(self.h, float(e), _d(Y), float(Tguess), _i(it))
         return T, int(it[0])
 
     def progress(self, C, T, jac=False):
@@ -139,29 +145,29 @@ class OracleMech:
         if jac:
             dqdC = np.zeros((self.R, self.N))
             dqdT = np.zeros(self.R)
-            lib().orc_progress(self.h, _d(C), float(T), _d(q), _d(dqdC), _d(dqdT))
+            self.L.orc_progress(self.h, _d(C), float(T), _d(q), _d(dqdC), _d(dqdT))
             return q, dqdC, dqdT
-        lib().orc_progress(self.h, _d(C), float(T), _d(q), None, None)
+        self.L.orc_progress(self.h, _d(C), float(T), _d(q), None, None)
         return q
 
     def wdot(self, C, T):
         C = np.ascontiguousarray(C, dtype=np.float64)
         w = np.zeros(self.N)
-        lib().orc_wdot(self.h, _d(C), float(T), _d(w))
+        self.L.orc_wdot(self.h, _d(C), float(T), _d(w))
         return w
 
     def rhs(self, t, y, rhoe0, rhoesrc=0.0, rYsrc=None):
         y = np.ascontiguousarray(y, dtype=np.float64)
         src = np.zeros(self.N) if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
         out = np.zeros(self.N + 1)
-        T = lib().orc_rhs(self.h, float(t), _d(y), float(rhoe0), float(rhoesrc), _d(src), _d(out))
+        T = self.L.orc_rhs(self.h, float(t), _d(y), float(rhoe0), float(rhoesrc), _d(src), _d(out))
         return out, T
 
     def jac(self, t, y, rhoe0, rhoesrc=0.0, rYsrc=None):
         y = np.ascontiguousarray(y, dtype=np.float64)
         src = np.zeros(self.N) if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
         J = np.zeros((self.N + 1, self.N + 1))
-        lib().orc_jac(self.h, float(t), _d(y), float(rhoe0), float(rhoesrc), _d(src), _d(J))
+        self.L.orc_jac(self.h, float(t), _d(y), float(rhoe0), float(rhoesrc), _d(src), _d(J))
         return J
 
     def react(self, rY, T, rE, dt, rYsrc=None, rEsrc=None, mask=None, rtol=1e-6, atol=1e-10,
@@ -179,7 +185,7 @@ class OracleMech:
         FC = np.zeros(nc)
         status = np.zeros(nc, dtype=np.int32)
         stats = np.zeros((nc, 7), dtype=np.int32)
-        nfail = lib().orc_react(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC), _i(msk),
+        nfail = self.L.orc_react(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC), _i(msk),
                                 float(dt), float(rtol), float(atol), _d(tv), int(max_steps),
                                 int(max_order), int(nthreads), _i(status), _i(stats))
         return {"rY": rY, "T": T, "rE": rE, "FC": FC, "status": status, "stats": stats, "nfail": nfail}
@@ -203,7 +209,7 @@ class OracleMech:
             dla = None if tangent[2] is None else np.ascontiguousarray(tangent[2], dtype=np.float64)
             dyend = np.zeros(n)
             dTend = np.zeros(1)
-        lib().orc_replay(self.h, _d(tape), tape.shape[0], tape.shape[1], _d(y0), float(rhoe0),
+        self.L.orc_replay(self.h, _d(tape), tape.shape[0], tape.shape[1], _d(y0), float(rhoe0),
                          float(rhoesrc), _d(src), _d(la), float(dt), _d(yend), _d(Tend), _d(dy0),
                          de if de is not None else 0.0, _d(dla), _d(dyend), _d(dTend))
         if tangent is None:
@@ -226,7 +232,7 @@ class OracleMech:
         FC = np.zeros(nc)
         status = np.zeros(nc, dtype=np.int32)
         stats = np.zeros((nc, 3), dtype=np.int32)
-        nfail = lib().orc_react_rk64(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC),
+        nfail = self.L.orc_react_rk64(self.h, nc, _d(rY), _d(src), _d(T), _d(rE), _d(esrc), _d(FC),
                                      _i(msk), float(dt), float(rtol), float(atol), _d(tv),
                                      int(nsub_guess), int(nsub_min), int(nsub_max), int(max_steps),
                                      int(nthreads), _i(status), _i(stats))

@@ -8,9 +8,10 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
-from typing import Iterable, List
+from typing import Dict, Iterable, List
 
 from . import mechgen
 from . import mechanism as M
@@ -30,7 +31,47 @@ HIP_FLAGS = [
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # per-kernel register / scratch / LDS figures, checked below
+    "-Rpass-analysis=kernel-resource-usage",
 ]
+
+#: Largest private segment (bytes per lane) a kernel may need. The runtime backs scratch for every
+#: wave the GPU can hold: 29.6 KB per lane (the plain kernel at drm19, n = 22) allocates and runs
+#: on 262144 cells, 64 KB per lane (the plain kernel at n = 41 with its matrices in private
+#: memory) was refused with HSA_STATUS_ERROR_OUT_OF_RESOURCES. A build whose kernels need more
+#: than the tested size fails here, on the CPU.
+MAX_SCRATCH_PER_LANE = 32 * 1024
+
+_REMARK = re.compile(r"remark:\s+(?:Function Name: (\S+)|([A-Za-z ]+?)(?: \[[^\]]*\])?: "
+                     r"(\S+)) \[-Rpass-analysis=kernel-resource-usage\]")
+
+
+def kernel_resources(stderr: str) -> Dict[str, Dict[str, str]]:
+    """Per-kernel resource figures from hipcc's kernel-resource-usage remarks (device pass)."""
+    out: Dict[str, Dict[str, str]] = {}
+    cur = None
+    for m in _REMARK.finditer(stderr):
+        if m.group(1):
+            cur = out.setdefault(m.group(1), {})
+        elif cur is not None:
+            cur[m.group(2).strip()] = m.group(3)
+    return out
+
+
+def check_resources(mech: str, res: Dict[str, Dict[str, str]]) -> List[str]:
+    """Kernels whose private segment exceeds MAX_SCRATCH_PER_LANE or that need a dynamic stack."""
+    bad = []
+    for fn, r in res.items():
+        if "ScratchSize" not in r:
+            continue
+        if int(r["ScratchSize"]) > MAX_SCRATCH_PER_LANE or r.get("Dynamic Stack") == "True":
+            bad.append(f"{mech}: {fn} scratch {r['ScratchSize']} B/lane, dynamic stack "
+                       f"{r.get('Dynamic Stack')}")
+    return bad
+
+
+def resources_path(mech: str) -> str:
+    return os.path.join(LIBDIR, f"libpelec_amd_{mech}.resources.txt")
 
 
 def lib_path(mech: str) -> str:
@@ -55,8 +96,21 @@ def build_one(mech: str, verbose: bool = False) -> str:
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {mech}:\n{res.stderr[-4000:]}")
-    if verbose and res.stderr:
-        sys.stderr.write(res.stderr)
+    kr = kernel_resources(res.stderr)
+    if not kr:
+        raise RuntimeError(f"{mech}: no kernel-resource-usage remarks in hipcc's output")
+    with open(resources_path(mech), "w") as f:
+        for fn in sorted(kr):
+            r = kr[fn]
+            f.write(f"{fn} VGPRs {r.get('VGPRs')} AGPRs {r.get('AGPRs')} "
+                    f"scratch {r.get('ScratchSize')} LDS {r.get('LDS Size')} "
+                    f"occupancy {r.get('Occupancy')}\n")
+    bad = check_resources(mech, kr)
+    if bad:
+        os.remove(tmp)
+        raise RuntimeError("private segment above the limit the runtime can back:\n" + "\n".join(bad))
+    if verbose:
+        sys.stderr.write(open(resources_path(mech)).read())
     os.replace(tmp, out)
     return out
 

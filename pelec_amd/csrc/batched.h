@@ -3,13 +3,16 @@
 //
 // Work is split by phase so a wave never executes another cell's rare branch:
 //   k_start   inputs -> state, f(t0,y0), initial step (cvHin), first attempt      (once per cell)
+//             for a range of free slots: all of them at first, then, whenever compaction frees
+//             enough slots, the next cells of the call (the queue refills, so a call may hold
+//             more cells than there are slots)
 //   k_jac     analytic Jacobian of the cells in the set-up list that need one       (~2 per cell)
 //   k_lu_rows Newton matrix I - gamma J factorised with one row per lane, one 32- or 64-lane
 //             group per matrix                                                      (~10 per cell)
 //   k_newton  one Newton correction for every cell in the Newton phase: RHS, LU solve, convergence
 //             test, and -- when the correction converged -- the error test, step completion,
 //             order/step choice and the next step's prediction                      (~60 per cell)
-//   k_compact repacks live cells into a dense slot range when half the slots are done.
+//   k_compact repacks live cells into a dense slot range when enough slots are done.
 // Cells that need a Newton-matrix set-up are appended (wave-aggregated) to a rotating list consumed
 // by k_jac/k_lu of the next iteration. Within an iteration the set-up cells and the Newton cells
 // are disjoint, so k_jac -> k_lu and k_newton run concurrently (two branches of the graph). The
@@ -19,7 +22,9 @@
 // Storage: per-slot structure-of-arrays for vectors and scalars (coalesced: a wave's lanes hold
 // consecutive slots); per-cell array-of-structures for the n x n Jacobian and LU factors, so the
 // scattered cells of a set-up batch write whole contiguous blocks and the Newton lanes stream
-// their own block.
+// their own block. A cell takes one of the C matrix blocks when it starts and keeps it while
+// compaction moves its slot; a finished cell returns its block to a free ring that the next
+// refill draws from.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,7 +57,8 @@ enum Ctr : int {
   CT_NACTIVE = 6,  // slots in use [0, nactive)
   CT_CUR = 7,      // state buffer holding the live slots (0/1)
   CT_NEWACTIVE = 8,
-  CT_COUNT = 9
+  CT_FTAIL = 9,    // matrix blocks returned to the free ring (its tail; the host tracks the head)
+  CT_COUNT = 10
 };
 
 __host__ __device__ constexpr int list_next(int r) { return r == NLISTS - 1 ? 0 : r + 1; }
@@ -67,6 +73,8 @@ struct StateBuf {
   double* src;      // N * C  (rhoY sources) or null
   int32_t* cell;    // C  slot -> global cell index
   int32_t* phase;   // C
+  int32_t* mb;      // C  slot -> the cell's matrix block (J, LU factors, pivots), kept for the
+                    //    cell's lifetime while compaction moves its slot
 };
 
 template <int n>
@@ -173,6 +181,21 @@ __device__ inline void wave_append(int32_t* list, unsigned long long* cnt, bool 
   if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = val;
 }
 
+// Wave-aggregated push of `val` onto a ring of C entries whose tail counter is *tail (monotonic;
+// entry i lives at ring[i % C]). The ring never holds more than C entries: each entry is a matrix
+// block, and there are C blocks.
+__device__ inline void wave_push_ring(int32_t* ring, unsigned long long* tail, int64_t C, bool pred,
+                                      int32_t val) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(tail, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) ring[(base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull))) % C] = val;
+}
+
 __device__ inline void wave_count(unsigned long long* cnt, bool pred) {
   const unsigned long long mask = __ballot(pred);
   if (mask == 0ull) return;
@@ -260,12 +283,30 @@ __device__ inline double lu_solve_group(const T (&row)[n], int perm_l, double b,
   return l < n ? double(x) : 0.0;
 }
 
+// Equilibration of the FP32 Newton matrix: component i is scaled by d_i, the power of two at or
+// below its error weight ewt_i (clamped to 2^-60..2^60), and the factors are those of D M D^-1.
+// Powers of two make the scaling exact in FP64, so only the FP32 cast of D M D^-1 rounds. In the
+// error-weighted variables the Newton convergence test measures, the scaled matrix is well
+// conditioned where M itself (rho Y_k ~ 1e-4 beside T ~ 1e3) is not.
+__device__ inline double pow2_scale(double w) {
+  const int k = ilogb(w);
+  return ldexp(1.0, k < -60 ? -60 : (k > 60 ? 60 : k));
+}
+
 // x = (LU)^{-1} P b with factors from lu_rows (AoS block `lu`, inverted diagonal). b in/out (FP64),
 // arithmetic in T. `gb` is this lane's LDS scratch (stride 64) used to gather b by the permutation.
+// With `dsc` (the equilibration of an FP32 matrix, one power of two per component) the factors
+// are those of D M D^-1: b is scaled by D on the way in and x by D^-1 on the way out.
 template <int n, typename T>
-__device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b, double* gb) {
+__device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b, double* gb,
+                                     const double* dsc = nullptr) {
+  if (dsc) {
 #pragma unroll
-  for (int i = 0; i < n; ++i) gb[i * 64] = b[i];
+    for (int i = 0; i < n; ++i) gb[i * 64] = b[i] * dsc[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < n; ++i) gb[i * 64] = b[i];
+  }
   T x[n];
 #pragma unroll
   for (int i = 0; i < n; ++i) x[i] = T(gb[perm[i] * 64]);
@@ -287,8 +328,13 @@ __device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b
     x[i] = s * lu[i * n + i];
     PELEC_MEMORY_FENCE();
   }
+  if (dsc) {
 #pragma unroll
-  for (int i = 0; i < n; ++i) b[i] = double(x[i]);
+    for (int i = 0; i < n; ++i) b[i] = double(x[i]) / dsc[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < n; ++i) b[i] = double(x[i]);
+  }
 }
 
 }  // namespace pelec

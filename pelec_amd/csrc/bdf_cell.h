@@ -83,9 +83,10 @@ struct NoTapeRec {
 // POLISH: after the modified Newton iteration converges, the corrector equation is solved to
 // machine precision with the exact Jacobian (the discrete adjoint linearises the exactly solved
 // step map, so its forward trajectory must be that map's). EXTMAT: the n x n matrices live in
-// caller-provided device memory (savedJ, M, Jx point there) instead of private memory.
+// caller-provided device memory (savedJ, M, Jx point there) instead of private memory. NREF: FP64
+// refinement steps after each solve with FP32 factors (unused with FP64 factors).
 template <class Mech, typename LUT, class Tape = NoTapeRec, bool MUL = false, bool POLISH = false,
-          bool EXTMAT = false>
+          bool EXTMAT = false, int NREF = 1>
 struct BdfCell {
   using Model = CellModel<Mech>;
   static constexpr int N = Mech::NSP;
@@ -154,7 +155,8 @@ struct BdfCell {
     return lu_factor<n, LUT>(M, piv);
   }
 
-  // Solve (I - gammap J) x = b in place; FP32 factors get one FP64 refinement step.
+  // Solve (I - gammap J) x = b in place; FP32 factors get NREF FP64 refinement steps
+  // (r = b - (x - gammap J x) with the FP64 Jacobian, x += M^-1 r).
   __device__ void solve(double* b) {
     if constexpr (sizeof(LUT) == sizeof(double)) {
       lu_solve<n, LUT>(M, piv, b);
@@ -163,14 +165,18 @@ struct BdfCell {
 #pragma unroll
       for (int i = 0; i < n; ++i) x[i] = b[i];
       lu_solve<n, LUT>(M, piv, x);
-      for (int i = 0; i < n; ++i) {
-        double jx = 0.0;
-        for (int j = 0; j < n; ++j) jx += savedJ[i * n + j] * x[j];
-        r[i] = b[i] - (x[i] - S.gammap * jx);
-      }
-      lu_solve<n, LUT>(M, piv, r);
+      for (int it = 0; it < NREF; ++it) {
+        for (int i = 0; i < n; ++i) {
+          double jx = 0.0;
+          for (int j = 0; j < n; ++j) jx += savedJ[i * n + j] * x[j];
+          r[i] = b[i] - (x[i] - S.gammap * jx);
+        }
+        lu_solve<n, LUT>(M, piv, r);
 #pragma unroll
-      for (int i = 0; i < n; ++i) b[i] = x[i] + r[i];
+        for (int i = 0; i < n; ++i) x[i] += r[i];
+      }
+#pragma unroll
+      for (int i = 0; i < n; ++i) b[i] = x[i];
     }
   }
 

@@ -19,6 +19,24 @@
 
 namespace pelec {
 
+// Mechanisms above this ODE size (the 53-species ones) call their production rates and rate
+// Jacobian as out-of-line device functions: the straight-line kinetics of 229-325 reactions,
+// inlined into every kernel that evaluates the model, made one library take tens of minutes to
+// compile. Each kernel now calls one compiled copy; the arrays cross the call in private memory.
+constexpr int KIN_INLINE_MAX_N = 32;
+
+template <class Mech, bool MUL>
+__device__ __noinline__ void wdot_outline(const double* C, double T, double* wd,
+                                          const double* rmul) {
+  Mech::template wdot<MUL>(C, T, wd, rmul);
+}
+
+template <class Mech, int n, typename JT, bool MUL>
+__device__ __noinline__ void wdot_jac_outline(const double* C, double T, double* wd, JT* J,
+                                              double* dwdT, const double* rmul) {
+  Mech::template wdot_jac<n, JT, MUL>(C, T, wd, J, dwdT, rmul);
+}
+
 template <class Mech>
 struct CellModel {
   static constexpr int N = Mech::NSP;
@@ -70,7 +88,10 @@ struct CellModel {
     }
     const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, Y, y[N]);
     double wd[N];
-    Mech::template wdot<MUL>(C, T, wd, rmul);
+    if constexpr (n > KIN_INLINE_MAX_N)
+      wdot_outline<Mech, MUL>(C, T, wd, rmul);
+    else
+      Mech::template wdot<MUL>(C, T, wd, rmul);
     double ek[N], cvk[N];
     Mech::e_cv(T, ek, cvk);
     double num = rhoesrc, D = 0.0;
@@ -105,7 +126,10 @@ struct CellModel {
     }
     const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, Y, y[N]);
     double wd[N], dwdT[N];
-    Mech::template wdot_jac<n, JT, MUL>(C, T, wd, J, dwdT, rmul);
+    if constexpr (n > KIN_INLINE_MAX_N)
+      wdot_jac_outline<Mech, n, JT, MUL>(C, T, wd, J, dwdT, rmul);
+    else
+      Mech::template wdot_jac<n, JT, MUL>(C, T, wd, J, dwdT, rmul);
     double ek[N], cvk[N], dcv[N];
     Mech::e_cv(T, ek, cvk);
     Mech::dcv_dT(T, dcv);

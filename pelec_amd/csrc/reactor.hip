@@ -79,7 +79,9 @@ PhaseKernels phase_kernels(const Shape& sh, bool tail = false) {
   if constexpr (sizeof(LUT) == sizeof(double))
     return phase_kernels_r<LUT, 1>(sh, tail);
   else
-    return sh.refine <= 1 ? phase_kernels_r<LUT, 1>(sh, tail) : phase_kernels_r<LUT, 2>(sh, tail);
+    return sh.refine <= 0   ? phase_kernels_r<LUT, 0>(sh, tail)
+           : sh.refine == 1 ? phase_kernels_r<LUT, 1>(sh, tail)
+                            : phase_kernels_r<LUT, 2>(sh, tail);
 }
 }
 
@@ -93,6 +95,7 @@ struct prc_reactor {
   int max_order = 5;
   int form = 0;
   int lu_fp32 = 0;
+  int max_slots = 0;  // prc.max_slots: slots of the batched form (0: one per cell, memory allowing)
   double typ[NV];
   bool have_typ = false;
   double* d_atol = nullptr;
@@ -106,7 +109,8 @@ struct prc_reactor {
   // batched workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
-  int64_t ws_cap = 0;
+  int64_t ws_cap = 0;   // slots
+  int64_t ws_req = 0;   // slots asked for when it was allocated
   int ws_lu_fp32 = -1;
   int ws_src = -1;
   Args ws_layout{};
@@ -191,64 +195,70 @@ bool box_ok(const BoxDev& b) { return b.ncells == 0 || (b.rY.p && b.T.p && b.rE.
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Batched workspace for C cells: two slot-state buffers, per-cell J / LU / perm, lists, remap.
-int ensure_workspace(prc_reactor* r, int64_t C, bool has_src) {
+// Layout of the batched workspace for C slots (and C matrix blocks) at base: two slot-state
+// buffers, per-block J / LU / pivots, the set-up lists, the compaction remap and the free ring of
+// matrix blocks. Returns the bytes it spans; with base == nullptr only the size is computed.
+size_t workspace_layout(Args& L, char* base, int64_t C, bool lu_fp32, bool has_src) {
+  const size_t lub = lu_fp32 ? sizeof(float) : sizeof(double);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align_up(bytes);
+    return base + o;
+  };
+  for (int b = 0; b < 2; ++b) {
+    StateBuf& st = L.st[b];
+    st.zn = reinterpret_cast<double*>(take(sizeof(double) * (BDF_QMAX + 1) * NV * C));
+    st.acor = reinterpret_cast<double*>(take(sizeof(double) * NV * C));
+    st.ewt = reinterpret_cast<double*>(take(sizeof(double) * NV * C));
+    st.sc = reinterpret_cast<uint64_t*>(take(sizeof(uint64_t) * BDF_SCALAR_WORDS * C));
+    st.rhoe = reinterpret_cast<double*>(take(sizeof(double) * 2 * C));
+    st.src = has_src ? reinterpret_cast<double*>(take(sizeof(double) * NSP * C)) : nullptr;
+    st.cell = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+    st.phase = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+    st.mb = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+  }
+  L.C = C;
+  L.J = reinterpret_cast<double*>(take(sizeof(double) * NV * NV * C));
+  L.LU = take(lub * NV * NV * C);
+  L.perm = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * NV * C));
+  L.dsc = lu_fp32 ? reinterpret_cast<double*>(take(sizeof(double) * NV * C)) : nullptr;
+  for (int b = 0; b < NLISTS; ++b) L.list[b] = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+  L.remap = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+  L.ring = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+  return off;
+}
+
+// Batched workspace for a call of ncells cells: one slot per cell up to prc.max_slots (0: no
+// limit) and up to half of the device memory free at the time; a call with more cells than slots
+// refills freed slots from the cell queue (run_batched).
+int ensure_workspace(prc_reactor* r, int64_t ncells, bool has_src) {
   const int want_fp32 = r->lu_fp32;
-  if (r->ws && C <= r->ws_cap && r->ws_lu_fp32 == want_fp32 && r->ws_src >= (int)has_src)
-    return PRC_OK;
+  int64_t want = std::max<int64_t>(ncells, 64);
+  if (r->max_slots > 0) want = std::min<int64_t>(want, std::max(r->max_slots, 64));
+  // an existing workspace serves when it holds every cell, or when it was already as large as the
+  // free memory allowed (the call then refills its slots)
+  const bool enough = want <= r->ws_cap || (r->ws_cap < r->ws_req && want >= r->ws_req);
+  if (r->ws && enough && r->ws_lu_fp32 == want_fp32 && r->ws_src >= (int)has_src) return PRC_OK;
   if (r->ws) {
     (void)hipFree(r->ws);
     r->ws = nullptr;
     r->ws_cap = 0;
   }
-  const int64_t cap = std::max<int64_t>(C, 64);
-  const size_t lub = want_fp32 ? sizeof(float) : sizeof(double);
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    const size_t o = off;
-    off += align_up(bytes);
-    return o;
-  };
-  size_t o_st[2][8];
-  for (int b = 0; b < 2; ++b) {
-    o_st[b][0] = take(sizeof(double) * (BDF_QMAX + 1) * NV * cap);
-    o_st[b][1] = take(sizeof(double) * NV * cap);
-    o_st[b][2] = take(sizeof(double) * NV * cap);
-    o_st[b][3] = take(sizeof(uint64_t) * BDF_SCALAR_WORDS * cap);
-    o_st[b][4] = take(sizeof(double) * 2 * cap);
-    o_st[b][5] = has_src ? take(sizeof(double) * NSP * cap) : 0;
-    o_st[b][6] = take(sizeof(int32_t) * cap);
-    o_st[b][7] = take(sizeof(int32_t) * cap);
-  }
-  const size_t o_J = take(sizeof(double) * NV * NV * cap);
-  const size_t o_LU = take(lub * NV * NV * cap);
-  const size_t o_perm = take(sizeof(int32_t) * NV * cap);
-  size_t o_l[NLISTS];
-  for (int b = 0; b < NLISTS; ++b) o_l[b] = take(sizeof(int32_t) * cap);
-  const size_t o_remap = take(sizeof(int32_t) * cap);
+  Args L{};
+  const size_t per_slot = workspace_layout(L, nullptr, 1024, want_fp32, has_src) / 1024 + 1;
+  size_t free_b = 0, total_b = 0;
+  HIP_OK(hipMemGetInfo(&free_b, &total_b));
+  const int64_t fit = (int64_t)(free_b / 2 / per_slot) & ~int64_t(63);
+  const int64_t cap = std::max<int64_t>(std::min(want, fit), 64);
+  const size_t bytes = workspace_layout(L, nullptr, cap, want_fp32, has_src);
   void* p = nullptr;
-  if (hipMalloc(&p, off) != hipSuccess) return PRC_ENOMEM;
-  char* base = static_cast<char*>(p);
-  Args& L = r->ws_layout;
-  for (int b = 0; b < 2; ++b) {
-    L.st[b].zn = reinterpret_cast<double*>(base + o_st[b][0]);
-    L.st[b].acor = reinterpret_cast<double*>(base + o_st[b][1]);
-    L.st[b].ewt = reinterpret_cast<double*>(base + o_st[b][2]);
-    L.st[b].sc = reinterpret_cast<uint64_t*>(base + o_st[b][3]);
-    L.st[b].rhoe = reinterpret_cast<double*>(base + o_st[b][4]);
-    L.st[b].src = has_src ? reinterpret_cast<double*>(base + o_st[b][5]) : nullptr;
-    L.st[b].cell = reinterpret_cast<int32_t*>(base + o_st[b][6]);
-    L.st[b].phase = reinterpret_cast<int32_t*>(base + o_st[b][7]);
-  }
-  L.C = cap;
-  L.J = reinterpret_cast<double*>(base + o_J);
-  L.LU = base + o_LU;
-  L.perm = reinterpret_cast<int32_t*>(base + o_perm);
-  for (int b = 0; b < NLISTS; ++b) L.list[b] = reinterpret_cast<int32_t*>(base + o_l[b]);
-  L.remap = reinterpret_cast<int32_t*>(base + o_remap);
+  if (hipMalloc(&p, bytes) != hipSuccess) return PRC_ENOMEM;
+  workspace_layout(r->ws_layout, static_cast<char*>(p), cap, want_fp32, has_src);
   r->ws = p;
-  r->ws_bytes = off;
+  r->ws_bytes = bytes;
   r->ws_cap = cap;
+  r->ws_req = want;
   r->ws_lu_fp32 = want_fp32;
   r->ws_src = has_src;
   return PRC_OK;
@@ -260,10 +270,11 @@ int newton_grid(const prc_reactor* r, bool tail) {
   return r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
 }
 
+// One graph's worth of batched iterations enqueued on s (set-up branch) and s2 (Newton branch),
+// forked and joined every iteration; the kernels read the call's arguments at `args`.
 template <typename LUT>
-int capture_one(prc_reactor* r, bool tail, hipGraphExec_t* out) {
-  hipStream_t s = r->cap_stream, s2 = r->cap_stream2;
-  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+bool enqueue_iterations(prc_reactor* r, hipStream_t s, hipStream_t s2, const Args* args,
+                        bool tail) {
   const int gj = r->ncu * (r->shape.bpc_jac > 0 ? r->shape.bpc_jac : r->occ_jac);
   const int gl = r->ncu * (r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu);
   const int gn = newton_grid(r, tail);
@@ -275,12 +286,20 @@ int capture_one(prc_reactor* r, bool tail, hipGraphExec_t* out) {
     ok = hipEventRecord(r->ev_fork, s) == hipSuccess &&
          hipStreamWaitEvent(s2, r->ev_fork, 0) == hipSuccess;
     if (!ok) break;
-    hipLaunchKernelGGL(pk.newton, dim3(gn), dim3(64), 0, s2, r->d_args, par);
-    hipLaunchKernelGGL(pk.jac, dim3(gj), dim3(64), 0, s, r->d_args, par);
-    hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, r->d_args, par);
+    hipLaunchKernelGGL(pk.newton, dim3(gn), dim3(64), 0, s2, args, par);
+    hipLaunchKernelGGL(pk.jac, dim3(gj), dim3(64), 0, s, args, par);
+    hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, args, par);
     ok = hipEventRecord(r->ev_join, s2) == hipSuccess &&
          hipStreamWaitEvent(s, r->ev_join, 0) == hipSuccess;
   }
+  return ok && hipGetLastError() == hipSuccess;
+}
+
+template <typename LUT>
+int capture_one(prc_reactor* r, bool tail, hipGraphExec_t* out) {
+  hipStream_t s = r->cap_stream, s2 = r->cap_stream2;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const bool ok = enqueue_iterations<LUT>(r, s, s2, r->d_args, tail);
   hipGraph_t graph = nullptr;
   if (hipStreamEndCapture(s, &graph) != hipSuccess || !ok) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -377,76 +396,112 @@ int eager_accumulate(prc_reactor* r) {
   return PRC_OK;
 }
 
+// The batched form of one call. `dargs` / `dctr` are the device argument block and counters
+// (the handle's, or a caller-provided block). With `graphs`, iterations run as
+// replays of the handle's captured graphs, which read the handle's argument block; without, they
+// are enqueued directly (no graph is instantiated).
 template <typename LUT>
-int run_batched(prc_reactor* r, Args& a, hipStream_t s) {
+int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned long long* dctr,
+                bool graphs) {
   if (r->graph_lu_fp32 != r->lu_fp32 || !r->ncu || !(r->graph_shape == r->shape)) {
     const int e = occupancy_setup<LUT>(r);
     if (e) return e;
   }
-  HIP_OK(hipMemcpyAsync(r->d_args, &a, sizeof(Args), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, a.ncells);
-  const int64_t nb = (a.ncells + 63) / 64;
-  hipLaunchKernelGGL(k_start, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+  HIP_OK(hipMemcpyAsync(dargs, &a, sizeof(Args), hipMemcpyHostToDevice, s));
+  // the first fill: cells 0 .. count0-1 into slots 0 .. count0-1, each slot its own matrix block
+  const int64_t C = a.C;
+  const int64_t count0 = std::min(C, a.ncells);
+  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, dctr, count0);
+  hipLaunchKernelGGL(k_start, dim3((unsigned)((count0 + 63) / 64)), dim3(64), 0, s, dargs,
+                     (int64_t)0, count0, (int64_t)0, (int64_t)-1);
   HIP_OK(hipGetLastError());
-  // the captured graph only reads r->d_args, so it is independent of the call
+  int64_t next = count0;  // next cell of the call to start
+  int64_t head = 0;       // free-ring entries taken so far
+  // the argument block is read by every kernel that follows: `a` (host) stays untouched until the
+  // copy has completed
   HIP_OK(hipStreamSynchronize(s));
-  int e = capture_graph<LUT>(r);
-  if (e) return e;
+  int e = PRC_OK;
+  if (graphs) {
+    // the captured graphs only read r->d_args, so they are independent of the call
+    e = capture_graph<LUT>(r);
+    if (e) return e;
+  }
   const int gn = r->ncu * (r->shape.bpc_newton > 0 ? r->shape.bpc_newton : r->occ_newton);
   int64_t iters = 0;
-  // safety net: every cell ends within max_steps steps of a bounded number of attempts
-  const int64_t max_iters = (int64_t)(r->max_steps + 1) * 64 + 64;
-  int64_t nactive = a.ncells;
+  // safety net: every cell ends within max_steps steps of a bounded number of attempts, and the
+  // slots are refilled at most ncells / C times over
+  const int64_t max_iters = ((int64_t)(r->max_steps + 1) * 64 + 64) * (1 + a.ncells / C);
+  int64_t nactive = count0;
   r->kt_ms[0] = r->kt_ms[1] = r->kt_ms[2] = 0.0;
   r->kt_launches = 0;
   for (;;) {
     const bool tail = r->shape.tail_slots > 0 && nactive <= r->shape.tail_slots;
-    if (r->kernel_timing) {
+    if (!graphs) {
+      if (!enqueue_iterations<LUT>(r, s, r->cap_stream2, dargs, tail)) return PRC_EHIP;
+    } else if (r->kernel_timing) {
       e = eager_iterations<LUT>(r, s, tail);
       if (e) return e;
     } else {
       HIP_OK(hipGraphLaunch(tail ? r->graph_tail : r->graph_exec, s));
     }
     iters += r->shape.iters;
-    HIP_OK(hipMemcpyAsync(r->h_ctr, r->d_ctr, CT_COUNT * sizeof(unsigned long long),
+    HIP_OK(hipMemcpyAsync(r->h_ctr, dctr, CT_COUNT * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (r->kernel_timing) {
+    if (graphs && r->kernel_timing) {
       e = eager_accumulate(r);
       if (e) return e;
     }
     const int64_t started = (int64_t)r->h_ctr[CT_STARTED];
     const int64_t done = (int64_t)r->h_ctr[CT_DONE];
-    if (done >= started) break;
+    const bool pending = next < a.ncells;
+    if (done >= started && !pending) break;
     if (iters > max_iters) return PRC_ESTATE;
     const int64_t alive = started - done;
     if (r->trace)
-      std::fprintf(stderr, "[pelec_amd] iter %lld started %lld done %lld alive %lld slots %lld\n",
-                   (long long)iters, (long long)started, (long long)done, (long long)alive,
-                   (long long)nactive);
-    if (alive * 100 < nactive * r->shape.compact_pct && nactive > 4096) {
-      hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, r->d_args);
-      hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, r->d_args, 0);
-      hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, r->d_args);
+      std::fprintf(stderr, "[pelec_amd] iter %lld started %lld done %lld alive %lld slots %lld "
+                   "queued %lld\n", (long long)iters, (long long)started, (long long)done,
+                   (long long)alive, (long long)nactive, (long long)(a.ncells - next));
+    if (alive * 100 < nactive * r->shape.compact_pct && (nactive > 4096 || pending)) {
+      hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, dargs);
+      hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, dargs, 0);
+      hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, dargs);
       HIP_OK(hipGetLastError());
       nactive = alive;
+      if (pending) {
+        // refill the freed slots [alive, C) with the next cells; their matrix blocks come from
+        // the free ring, which holds C - alive blocks (every slot filled so far either holds a
+        // live cell or has returned its block)
+        const int64_t k = std::min(C - alive, a.ncells - next);
+        hipLaunchKernelGGL(k_start, dim3((unsigned)((k + 63) / 64)), dim3(64), 0, s, dargs,
+                           alive, k, next, head);
+        HIP_OK(hipGetLastError());
+        next += k;
+        head += k;
+        nactive = alive + k;
+      }
     }
   }
   r->last_iterations = iters;
   return PRC_OK;
 }
 
-// One react call: a.box (single box) or a.boxes / a.nboxes (device table), a.ncells cells.
-int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_counters = true) {
+// Integrator parameters, diagnostics and counters of the handle into the call's argument block.
+void fill_params(const prc_reactor* r, Args& a, unsigned long long* ctr) {
   a.prm.rtol = r->rtol;
   a.prm.atolv = r->d_atol;
   a.prm.qmax = std::min(std::max(r->max_order, 1), BDF_QMAX);
   a.prm.max_steps = r->max_steps;
   a.rk = Rk64Params{r->rtol, r->d_atol, r->rk_guess, r->rk_min, r->rk_max, r->max_steps};
-  if (r->integrator == 1 && r->rk_min > r->rk_max) return PRC_EINVAL;
   a.diag = r->diag;
   a.diag_cap = r->diag_cap;
-  a.ctr = r->d_ctr;
+  a.ctr = ctr;
+}
+
+// One react call: a.box (single box) or a.boxes / a.nboxes (device table), a.ncells cells.
+int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_counters = true) {
+  if (r->integrator == 1 && r->rk_min > r->rk_max) return PRC_EINVAL;
+  fill_params(r, a, r->d_ctr);
   if (a.ncells == 0 || r->form == 0 || r->integrator == 1) {
     if (reset_counters) {
       hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, (int64_t)0);
@@ -469,10 +524,12 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
     const int64_t nb = (a.box.ncells + 63) / 64;
     if (r->integrator == 1)
       hipLaunchKernelGGL(react_rk64, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+    else if (r->lu_fp32 && r->shape.refine >= 2)
+      hipLaunchKernelGGL((react_plain<float, 2>), dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
     else if (r->lu_fp32)
-      hipLaunchKernelGGL(react_plain<float>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+      hipLaunchKernelGGL((react_plain<float, 1>), dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
     else
-      hipLaunchKernelGGL(react_plain<double>, dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
+      hipLaunchKernelGGL((react_plain<double, 1>), dim3((unsigned)nb), dim3(64), 0, s, r->d_args);
     HIP_OK(hipGetLastError());
     // the argument block is reused by the next call: keep it until this launch has read it
     HIP_OK(hipStreamSynchronize(s));
@@ -487,11 +544,14 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   a.J = L.J;
   a.LU = L.LU;
   a.perm = L.perm;
+  a.dsc = L.dsc;
   for (int b = 0; b < NLISTS; ++b) a.list[b] = L.list[b];
   a.remap = L.remap;
+  a.ring = L.ring;
   a.has_src = has_src ? 1 : 0;
   a.test_lu_fail = r->test_lu_fail;
-  return r->lu_fp32 ? run_batched<float>(r, a, s) : run_batched<double>(r, a, s);
+  return r->lu_fp32 ? run_batched<float>(r, a, s, r->d_args, r->d_ctr, true)
+                    : run_batched<double>(r, a, s, r->d_args, r->d_ctr, true);
 }
 
 int ensure_init(prc_reactor* r) {
@@ -666,7 +726,7 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
     if (v < NLISTS || v > 300 || v % NLISTS) return PRC_EINVAL;
     r->shape.iters = v;
   } else if (!std::strcmp(key, "prc.lu_refine")) {
-    if (v < 1 || v > 2) return PRC_EINVAL;
+    if (v < 0 || v > 2) return PRC_EINVAL;
     r->shape.refine = v;
   } else if (!std::strcmp(key, "prc.cooperative")) {
     if (v < 0 || v > 3 || (v && !COOP)) return PRC_EINVAL;
@@ -689,6 +749,9 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "prc.compact_pct")) {
     if (v < 1 || v > 100) return PRC_EINVAL;
     r->shape.compact_pct = v;
+  } else if (!std::strcmp(key, "prc.max_slots")) {
+    if (v < 0) return PRC_EINVAL;
+    r->max_slots = v;
   } else if (!std::strcmp(key, "prc.kernel_timing")) {
     r->kernel_timing = v ? 1 : 0;
   } else if (!std::strcmp(key, "prc.test_lu_fail")) {
@@ -723,6 +786,10 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
     *v = r->lu_fp32;
   else if (!std::strcmp(key, "prc.lu_refine"))
     *v = r->shape.refine;
+  else if (!std::strcmp(key, "prc.max_slots"))
+    *v = r->max_slots;
+  else if (!std::strcmp(key, "prc.slots"))  // slots of the current batched workspace
+    *v = (int)r->ws_cap;
   else
     return PRC_EINVAL;
   return PRC_OK;
@@ -1091,6 +1158,7 @@ int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double*
   int e = ensure_init(r);
   if (e) return e;
   if (ncells < 0 || !(dt > 0.0) || r->integrator != 0) return PRC_EINVAL;
+  if (!ADJ_OK) return PRC_EINVAL;  // n above ADJ_MAX_N: no adjoint kernel for this mechanism
   if (ncells == 0) return PRC_OK;
   if (!rY0 || !T0 || !rE0 || !rY_out || !T_out || !g_rY0 || !g_rE0 || !g_logA || !status)
     return PRC_EINVAL;
@@ -1139,7 +1207,7 @@ int prc_adjoint(prc_reactor* r, int64_t ncells, const double* rY0, const double*
   a.full_count = tape_count;
   hipLaunchKernelGGL(k_exp, dim3((Mech::NRXN + 63) / 64), dim3(64), 0, s, (int)Mech::NRXN, logA,
                      rmul);
-  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((ncells + 63) / 64)), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_adjoint<NV>, dim3((unsigned)((ncells + 63) / 64)), dim3(64), 0, s, a);
   HIP_OK(hipGetLastError());
   return PRC_OK;
 }

@@ -62,14 +62,16 @@ struct Args {
   int32_t* diag;
   int64_t diag_cap;
   unsigned long long* ctr;  // counters (CT_*)
-  // batched workspace
+  // batched workspace: C slots and C matrix blocks
   StateBuf st[2];
   int64_t C;
-  double* J;       // per cell n*n (FP64)
-  void* LU;        // per cell n*n (LUT)
-  int32_t* perm;   // per cell n
+  double* J;       // per matrix block n*n (FP64)
+  void* LU;        // per matrix block n*n (LUT)
+  int32_t* perm;   // per matrix block n
+  double* dsc;     // per matrix block n: equilibration of FP32 factors (pow2_scale), else unused
   int32_t* list[NLISTS];
   int32_t* remap;
+  int32_t* ring;   // C: free matrix blocks (tail counter CT_FTAIL, head kept by the host)
   int has_src;
   int test_lu_fail;  // prc.test_lu_fail: every factorisation reports a singular matrix (tests)
   // plain form above PLAIN_PRIVATE_N: per cell of the box, its saved Jacobian and Newton matrix
@@ -161,7 +163,8 @@ __device__ inline void store_cell(const Args& a, int64_t g, int status, const Co
 constexpr int PLAIN_PRIVATE_N = 32;
 constexpr bool PLAIN_EXT = NV > PLAIN_PRIVATE_N;
 
-template <typename LUT>
+// NREF: FP64 refinement steps after each FP32-LU solve (prc.lu_refine; FP64 factors: unused)
+template <typename LUT, int NREF>
 __global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
   const Args& a = *ap;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,7 +178,7 @@ __global__ void __launch_bounds__(64) react_plain(const Args* __restrict__ ap) {
     write_diag(a, g, CELL_OK, nullptr);
     return;
   }
-  BdfCell<Mech, LUT, NoTapeRec, false, false, PLAIN_EXT> cell;
+  BdfCell<Mech, LUT, NoTapeRec, false, false, PLAIN_EXT, NREF> cell;
   if constexpr (PLAIN_EXT) {
     double* w = a.pws + c * (int64_t)(2 * NV * NV);
     cell.savedJ = w;
@@ -261,19 +264,30 @@ __device__ inline void rhs_slot(const Args& a, const StateBuf& st, int64_t s, do
   Model::rhs(t, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C, out);
 }
 
-// inputs -> state, f(t0, y0), cvHin, first attempt (always a set-up: nst == 0)
-__global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap) {
+// Cells g0 .. g0+count-1 into slots slot0 .. slot0+count-1 of the current state buffer:
+// inputs -> state, f(t0, y0), cvHin, first attempt (always a set-up: nst == 0). Matrix block of
+// slot slot0+t: the slot itself when head < 0 (the first fill), else free-ring entry head+t. A
+// cell that does not start (masked, bad input) returns its block at once.
+__global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap, int64_t slot0,
+                                              int64_t count, int64_t g0, int64_t head) {
   const Args& a = *ap;
-  const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  const StateBuf& st = a.st[0];
-  bool started = false;
-  if (g < a.ncells) {
+  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const StateBuf& st = a.st[a.ctr[CT_CUR]];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_NACTIVE] = (unsigned long long)(slot0 + count);
+  bool started = false, unused = false;
+  const int64_t g = g0 + t;
+  const int64_t slot = slot0 + t;
+  int32_t mblk = 0;
+  if (t < count) {
+    mblk = head < 0 ? (int32_t)slot : a.ring[(head + t) % a.C];
     int64_t loc;
     const BoxDev& b = box_of(a, g, loc);
     int i, j, k;
     cell_ijk(b, loc, i, j, k);
-    st.cell[g] = (int32_t)g;
-    st.phase[g] = PH_DONE;
+    st.cell[slot] = (int32_t)g;
+    st.phase[slot] = PH_DONE;
+    st.mb[slot] = mblk;
+    unused = true;
     if (b.mask.p && b.mask.at(i, j, k) == 0) {
       if (b.FC.p) b.FC.at(i, j, k, 0) = 0.0;
       write_diag(a, g, CELL_OK, nullptr);
@@ -292,22 +306,23 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap) {
         write_diag(a, g, CELL_BADINPUT, nullptr);
       } else {
         started = true;
-        st.rhoe[g] = b.rE.at(i, j, k, 0);
-        st.rhoe[a.C + g] = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
+        unused = false;
+        st.rhoe[slot] = b.rE.at(i, j, k, 0);
+        st.rhoe[a.C + slot] = b.rEsrc.p ? b.rEsrc.at(i, j, k, 0) : 0.0;
         if (a.has_src) {
 #pragma unroll
           for (int s = 0; s < NSP; ++s)
-            st.src[s * a.C + g] = b.rYsrc.p ? b.rYsrc.at(i, j, k, s) : 0.0;
+            st.src[s * a.C + slot] = b.rYsrc.p ? b.rYsrc.at(i, j, k, s) : 0.0;
         }
-        const SoAV<NV> v = soa(st, a.C, g);
+        const SoAV<NV> v = soa(st, a.C, slot);
         BdfScalars S;
         scalars_reset(S, a.prm.qmax);
 #pragma unroll
         for (int e = 0; e < NV; ++e) v.Z(0, e) = y0[e];
         set_ewt<NV>(v, a.prm);
-        auto f = [&](double t, const double* yy, double* out) {
+        auto f = [&](double tt, const double* yy, double* out) {
           S.nfe++;
-          rhs_slot(a, st, g, t, yy, out);
+          rhs_slot(a, st, slot, tt, yy, out);
         };
         double f0[NV];
         f(S.tn, y0, f0);
@@ -321,13 +336,14 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap) {
         for (int e = 0; e < NV; ++e) v.Z(1, e) *= S.h;
         step_prologue<NV>(S, v, a.prm.qmax);
         begin_attempt<NV>(S, v, a.dt);  // nst == 0: set-up required
-        store_scalars(S, st.sc, a.C, g);
-        st.phase[g] = PH_SETUP;
+        store_scalars(S, st.sc, a.C, slot);
+        st.phase[slot] = PH_SETUP;
       }
     }
   }
   wave_count(&a.ctr[CT_STARTED], started);
-  wave_append(a.list[0], &a.ctr[CT_SETUP0], started, (int32_t)g);
+  wave_append(a.list[0], &a.ctr[CT_SETUP0], started, (int32_t)slot);
+  wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, unused, mblk);
 }
 
 // Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision)
@@ -358,14 +374,14 @@ __global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par
     }
     PELEC_MEMORY_FENCE();
     if (need) {
-      const int64_t g = st.cell[s];
+      const int64_t mb = st.mb[s];
       const SoAV<NV> v = soa(st, a.C, s);
       double y[NV];
 #pragma unroll
       for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e);
       const double* src = a.has_src ? st.src + s : nullptr;
       Model::template jac<double>(tn, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
-                                  a.J + g * (int64_t)(NV * NV));
+                                  a.J + mb * (int64_t)(NV * NV));
     }
   }
 }
@@ -388,7 +404,7 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
     const int64_t idx = base + lane / G;
     const bool valid = idx < cnt;
     const int64_t s = valid ? a.list[par][idx] : 0;
-    const int64_t g = valid ? st.cell[s] : 0;
+    const int64_t g = valid ? st.mb[s] : 0;  // the cell's matrix block
     constexpr int W_GAMMA = offsetof(BdfScalars, gamma) / 8;
     double gamma = 0.0;
     if (valid) {
@@ -396,12 +412,25 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
       __builtin_memcpy(&gamma, &w, 8);
     }
     LUT row[NV];
+    constexpr bool EQ = sizeof(LUT) == sizeof(float);  // FP32 factors are equilibrated
+    double dr = 1.0;
     if (valid && r < NV) {
       const double* Jr = a.J + g * (int64_t)(NV * NV) + r * NV;
+      if constexpr (EQ) {
+        // D (I - gamma J) D^-1 with d_c the power of two of the cell's error weight ewt_c
+        const double* ew = st.ewt + s;
+        dr = pow2_scale(ew[(int64_t)r * a.C]);
 #pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        const LUT m = LUT(-gamma * Jr[c]);
-        row[c] = (c == r) ? m + LUT(1) : m;  // I - gamma J
+        for (int c = 0; c < NV; ++c) {
+          const double m = (c == r ? 1.0 : 0.0) - gamma * Jr[c];
+          row[c] = LUT(m * dr / pow2_scale(ew[(int64_t)c * a.C]));
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+          const LUT m = LUT(-gamma * Jr[c]);
+          row[c] = (c == r) ? m + LUT(1) : m;  // I - gamma J
+        }
       }
     } else {
 #pragma unroll
@@ -416,6 +445,7 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
 #pragma unroll
       for (int c = 0; c < NV; ++c) LUg[c] = row[c];
       a.perm[g * NV + pos] = r;
+      if constexpr (EQ) a.dsc[g * NV + r] = dr;
     }
     if (valid && leader) {
       after_setup_slot(st.sc, a.C, s);
@@ -454,6 +484,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     if (act || lufail) {
       const SoAV<NV> v = soa(st, a.C, s);
       const int64_t g = st.cell[s];
+      const int64_t mb = st.mb[s];
       BdfScalars& S = scl[lane];
       int r = 3;
       if (lufail) {
@@ -479,19 +510,22 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
 #pragma unroll
           for (int e = 0; e < NV; ++e) v.A(e) = 0.0;
         }
-        const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV);
-        const int32_t* pm = a.perm + g * NV;
-        const double* Jg = a.J + g * (int64_t)(NV * NV);
+        const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV);
+        const int32_t* pm = a.perm + mb * NV;
+        const double* Jg = a.J + mb * (int64_t)(NV * NV);
+        const double* dsc = a.dsc + mb * NV;
         const double gammap = S.gammap;
         double* gbl = gb + lane;
         auto solve = [&](double* b) {
           if constexpr (sizeof(LUT) == sizeof(double)) {
             lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
+          } else if constexpr (NREF == 0) {
+            lu_solve_perm<NV, LUT>(lu, pm, b, gbl, dsc);
           } else {
             double x[NV], res[NV];
 #pragma unroll
             for (int e = 0; e < NV; ++e) x[e] = b[e];
-            lu_solve_perm<NV, LUT>(lu, pm, x, gbl);
+            lu_solve_perm<NV, LUT>(lu, pm, x, gbl, dsc);
 #pragma unroll
             for (int it = 0; it < NREF; ++it) {
 #pragma unroll
@@ -502,7 +536,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
                 res[e] = b[e] - (x[e] - gammap * jx);
                 PELEC_MEMORY_FENCE();
               }
-              lu_solve_perm<NV, LUT>(lu, pm, res, gbl);
+              lu_solve_perm<NV, LUT>(lu, pm, res, gbl, dsc);
 #pragma unroll
               for (int e = 0; e < NV; ++e) x[e] += res[e];
             }
@@ -574,6 +608,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup,
                 (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done);
+    wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, done, done ? st.mb[s] : 0);
   }
 }
 
@@ -646,10 +681,10 @@ __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int p
       if (l == 0) store_scalars(S, st.sc, a.C, s);
     }
     if (need) {
-      const int64_t g = st.cell[s];
+      const int64_t mb = st.mb[s];
       const double yl = l < NV ? st.zn[(int64_t)l * a.C + s] : 0.0;
       const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
-      GM::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + g * (int64_t)(NV * NV));
+      GM::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + mb * (int64_t)(NV * NV));
     }
   }
 }
@@ -677,6 +712,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
     if (act || lufail) {
       const GrpV<NV, GRP> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
       const int64_t g = st.cell[s];
+      const int64_t mb = st.mb[s];
       const double rhoe0 = st.rhoe[s], rhoesrc = st.rhoe[a.C + s];
       const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
       BdfScalars& S = scl[threadIdx.x / GRP];
@@ -704,11 +740,13 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
         auto solve = [&](LaneVal& b) {
           LUT row[NV];
           int pl = l;
+          double dl = 1.0;  // this lane's component of the equilibration (FP32 factors)
           if (l < NV) {
-            const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + g * (int64_t)(NV * NV) + l * NV;
+            const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV) + l * NV;
 #pragma unroll
             for (int c = 0; c < NV; ++c) row[c] = lr[c];
-            pl = a.perm[g * NV + l];
+            pl = a.perm[mb * NV + l];
+            if constexpr (sizeof(LUT) != sizeof(double)) dl = a.dsc[mb * NV + l];
           } else {
 #pragma unroll
             for (int c = 0; c < NV; ++c) row[c] = LUT(0);
@@ -716,9 +754,10 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
           if constexpr (sizeof(LUT) == sizeof(double)) {
             b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
           } else {
-            // FP32 factors, one FP64 refinement step against the FP64 Jacobian (k_newton's)
-            double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
-            const double* Jr = a.J + g * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
+            // FP32 factors of D M D^-1, then NREF FP64 refinement steps against the FP64
+            // Jacobian (k_newton's)
+            double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x * dl, l) / dl;
+            const double* Jr = a.J + mb * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
 #pragma unroll
             for (int it = 0; it < NREF; ++it) {
               double jx = 0.0;
@@ -728,7 +767,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
                 if (l < NV) jx += Jr[c] * xc;
               }
               const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
-              x += lu_solve_group<NV, LUT, GRP>(row, pl, rr, l);
+              x += lu_solve_group<NV, LUT, GRP>(row, pl, rr * dl, l) / dl;
             }
             b.x = x;
           }
@@ -796,6 +835,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup && l == 0,
                 (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done && l == 0);
+    wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, done && l == 0, done ? st.mb[s] : 0);
   }
 }
 
@@ -839,6 +879,7 @@ __global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap) {
       }
       dst.cell[d] = src.cell[s];
       dst.phase[d] = src.phase[s];
+      dst.mb[d] = src.mb[s];
     }
   }
 }
@@ -925,9 +966,17 @@ __device__ inline void ckpt_load(AdjCell& cell, const double* p) {
   (void)lu_factor<NV, double>(cell.M, cell.piv);
 }
 
+// The adjoint keeps one cell's Nordsieck history, tape records and adjoint state in private memory:
+// 24 KB per lane at n = 22 (drm19), 38 KB at n = 41, more than the runtime was shown to back.
+// Mechanisms above this size build the kernel with an empty body and prc_adjoint rejects them.
+constexpr int ADJ_MAX_N = 32;
+constexpr bool ADJ_OK = NV <= ADJ_MAX_N;
+
 // Forward integration with checkpoints, then the backward sweep segment by segment (re-running
 // each segment from its checkpoint with the tape on). One cell per lane.
+template <int NN>
 __global__ void __launch_bounds__(64) k_adjoint(const AdjArgs a) {
+  if constexpr (NN <= ADJ_MAX_N) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncells) return;
   const int64_t nc = a.ncells;
@@ -1045,6 +1094,7 @@ __global__ void __launch_bounds__(64) k_adjoint(const AdjArgs a) {
   for (int k = 0; k < NSP; ++k) a.g_rY0[k * nc + c] = adj.gy0[k];
   a.g_rE0[c] = adj.ge;
   for (int j = 0; j < Mech::NRXN; ++j) a.g_p[j * nc + c] = adj.gp[j];
+  }
 }
 
 // ------------------------------------------------------------------ react_state pack / unpack
@@ -1231,7 +1281,10 @@ __global__ void eos_rty2wdot(int64_t nc, const double* rho, const double* T, con
   double C[NSP], wd[NSP];
 #pragma unroll
   for (int s = 0; s < NSP; ++s) C[s] = rho[c] * Y[s * nc + c] * Mech::IW[s];
-  Mech::wdot(C, T[c], wd);
+  if constexpr (NV > KIN_INLINE_MAX_N)
+    wdot_outline<Mech, false>(C, T[c], wd, nullptr);
+  else
+    Mech::wdot(C, T[c], wd);
 #pragma unroll
   for (int s = 0; s < NSP; ++s) wdot[s * nc + c] = wd[s];
 }

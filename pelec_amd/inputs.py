@@ -12,10 +12,15 @@ Workloads (species order of the mechanism files):
   drm19_64:   drm19 CH4/air, 64^3 cells, T ~ U[900,2200] K, p ~ U[1,10] atm, phi ~ U[0.6,1.6];
               10 % of cells ignition-seeded: T ~ U[1400,1600] K with Y_H = Y_OH = 1e-4.
   drm19_adj:  drm19 CH4/air, 16384 cells, T ~ U[1000,1600] K, p = 1 atm, phi ~ U[0.5,2].
-  drm19_amr:  an AMR-shaped batch (make_amr_batch): boxes with edge lengths drawn from
-              {8, 16, 32, 48, 64} per axis, 20 % of the cells masked out, T ~ U[800,2400] K,
-              p ~ U[1,20] atm, phi ~ U[0.5,1.8], dt = 5e-6 s; drm19 CH4/air.
-Air is O2 + 3.76 N2 (no Ar); H2/air uses n_H2 = 2 phi, CH4/air n_CH4 = phi / 2 per O2.
+  dodecane_64: dodecane_lu n-C12H26/air, 64^3 cells, T ~ U[700,1800] K, p ~ U[10,60] atm,
+              phi ~ U[0.5,2] (n_C12H26 = phi / 18.5 per O2), dt = 1e-5 s, rtol 1e-6.
+  gri30_amr:  GRI-Mech 3.0 CH4/air AMR-shaped batch (make_amr_batch): boxes with edge lengths
+              drawn from {8, 16, 32, 48, 64} per axis, 20 % of the cells masked out,
+              T ~ U[800,2400] K, p ~ U[1,20] atm, phi ~ U[0.5,1.8], dt = 5e-6 s.
+  drm19_amr:  the same batch with drm19 (same boxes, mask and states, drm19 species).
+  gri30_flat: cells of the gri30_amr state distribution, no mask, as flat arrays.
+Air is O2 + 3.76 N2 (no Ar); H2/air uses n_H2 = 2 phi, CH4/air n_CH4 = phi / 2 per O2,
+n-dodecane/air n_C12H26 = phi / 18.5 per O2.
 """
 from __future__ import annotations
 
@@ -85,6 +90,8 @@ WORKLOADS: Dict[str, Workload] = {
     "lidryer_32": Workload("lidryer_32", "LiDryer", 32 ** 3, 1e-6, 1e-10, 1e-10, 1001),
     "drm19_64": Workload("drm19_64", "drm19", 64 ** 3, 1e-5, 1e-6, 1e-10, 2002),
     "drm19_adj": Workload("drm19_adj", "drm19", 16384, 1e-4, 1e-6, 1e-10, 4004),
+    "gri30_flat": Workload("gri30_flat", "gri30", 262144, 5e-6, 1e-6, 1e-10, 5005),
+    "dodecane_64": Workload("dodecane_64", "dodecane_lu", 64 ** 3, 1e-5, 1e-6, 1e-10, 3003),
 }
 
 
@@ -126,6 +133,16 @@ def make_inputs(workload: str, ncells: Optional[int] = None, first: int = 0) -> 
         for sp in ("H", "OH"):
             Y[ign, mech.species_index(sp)] = 1e-4
         Y /= Y.sum(-1, keepdims=True)
+    elif wl.name == "dodecane_64":
+        T = 700.0 + 1100.0 * u(0)
+        p = ATM * (10.0 + 50.0 * u(1))
+        phi = 0.5 + 1.5 * u(2)
+        Y = _fuel_air_Y(mech, "NC12H26", phi / 18.5)
+    elif wl.name == "gri30_flat":
+        T = 800.0 + 1600.0 * u(0)
+        p = ATM * (1.0 + 19.0 * u(1))
+        phi = 0.5 + 1.3 * u(2)
+        Y = _fuel_air_Y(mech, "CH4", 0.5 * phi)
     elif wl.name == "drm19_adj":
         T = 1000.0 + 600.0 * u(0)
         p = np.full(n, ATM)
@@ -175,13 +192,18 @@ def make_conservative_state(wl, n, seed=7, hydro=True, ufs=7):
 
 
 AMR_EDGES = (8, 16, 32, 48, 64)
-AMR = Workload("drm19_amr", "drm19", 0, 5e-6, 1e-6, 1e-10, 5005)
+AMR_WORKLOADS = {
+    "gri30": Workload("gri30_amr", "gri30", 0, 5e-6, 1e-6, 1e-10, 5005),
+    "drm19": Workload("drm19_amr", "drm19", 0, 5e-6, 1e-6, 1e-10, 5005),
+}
+AMR = AMR_WORKLOADS["drm19"]
 
 
-def make_amr_batch(nboxes: int = 256) -> Dict[str, object]:
-    """The drm19_amr batch: per box (lo, hi, nx, ny, nz, first cell); all cells concatenated box
-    by box, x fastest, as rY (N, ncells), T, rE and mask (int32, 0 = covered: not integrated)."""
-    wl = AMR
+def make_amr_batch(nboxes: int = 256, mech_name: str = "drm19") -> Dict[str, object]:
+    """The <mech>_amr batch: per box (lo, hi, nx, ny, nz, first cell); all cells concatenated box
+    by box, x fastest, as rY (N, ncells), T, rE and mask (int32, 0 = covered: not integrated).
+    Boxes, mask, T, p and phi do not depend on the mechanism."""
+    wl = AMR_WORKLOADS[mech_name]
     mech = M.load(wl.mech)
     th = Thermo(mech)
     bi = np.arange(nboxes, dtype=np.int64)

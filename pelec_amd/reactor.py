@@ -192,11 +192,12 @@ class Reactor:
 
     def configure(self, rtol=None, atol=None, max_order=None, max_steps=None, form=None,
                   lu_fp32=None, kernel_timing=None, rk64_nsubsteps=None, cooperative=None,
-                  lu_refine=None):
+                  lu_refine=None, max_slots=None):
         """rk64_nsubsteps: (guess, min, max) substep counts of ReactorRK64. cooperative: which of
         the batched form's kernels run one cell per 32-lane group -- True / 3 both, 1 the Newton
         kernel, 2 the Jacobian kernel, False / 0 neither. lu_refine: FP64 refinement steps (1 or 2)
-        after each FP32-LU solve."""
+        after each FP32-LU solve. max_slots: slots of the batched form's work queue (0: one per
+        cell while device memory allows); a call with more cells refills freed slots."""
         if rtol is not None:
             self.set_param("ode.rtol", float(rtol))
         if atol is not None:
@@ -213,6 +214,8 @@ class Reactor:
             self.set_param("prc.kernel_timing", 1 if kernel_timing else 0)
         if lu_refine is not None:
             self.set_param("prc.lu_refine", int(lu_refine))
+        if max_slots is not None:
+            self.set_param("prc.max_slots", int(max_slots))
         if cooperative is not None:
             self.set_param("prc.cooperative",
                            3 if cooperative is True else int(cooperative))

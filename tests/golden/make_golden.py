@@ -61,6 +61,7 @@ def solve(wl_name, cells):
 PMF_FILES = {
     "drm19": "Exec/RegTests/PMF/PMF_CH4_1bar_300K_DRM_MixAvg.dat",
     "LiDryer": "Exec/RegTests/PMF/LiDryer_H2_p1_phi0_4000tu0300.dat",
+    "dodecane_lu": "Exec/RegTests/PMF/PMF_NC12H26_1bar_300K_DodecaneLu.dat",
 }
 
 
