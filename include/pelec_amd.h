@@ -100,7 +100,7 @@ int prc_set_typ_vals_ode(prc_reactor* r, const double* typ, int n);
  * of Exec/Production/ChallengeProblem/challenge.inp:29-31) and the gfx950 execution form:
  * "prc.form" = 0 plain FP64 one-cell-per-lane drop-in, 1 batched persistent work-queue kernel
  * (ReactorCvode only); "prc.lu_fp32" = 1 runs the Newton-matrix LU in FP32, each solve followed
- * by "prc.lu_refine" (0, 1 or 2, default 2) FP64 refinement steps against the FP64 Jacobian. In
+ * by "prc.lu_refine" (0, 1 or 2, default 1) FP64 refinement steps against the FP64 Jacobian. In
  * the batched form the FP32 factors are those of D M D^-1, D the powers of two of the cell's error
  * weights (exact scaling); the plain form factors M unscaled and always refines at least once. */
 int prc_set_param_real(prc_reactor* r, const char* key, double value);

@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 #: mechanisms built by default
-DEFAULT_MECHS = ("LiDryer", "drm19", "drm19_inert40")
+DEFAULT_MECHS = ("LiDryer", "drm19", "drm19_inert40", "gri30", "dodecane_lu")
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -36,11 +36,11 @@ HIP_FLAGS = [
 ]
 
 #: Largest private segment (bytes per lane) a kernel may need. The runtime backs scratch for every
-#: wave the GPU can hold: 29.6 KB per lane (the plain kernel at drm19, n = 22) allocates and runs
-#: on 262144 cells, 64 KB per lane (the plain kernel at n = 41 with its matrices in private
-#: memory) was refused with HSA_STATUS_ERROR_OUT_OF_RESOURCES. A build whose kernels need more
-#: than the tested size fails here, on the CPU.
-MAX_SCRATCH_PER_LANE = 32 * 1024
+#: wave the GPU can hold: 29,568 B per lane (the plain FP32-LU kernel at drm19, n = 22) allocates
+#: and runs; 30,528 B (the plain kernel at n = 41) and 64 KB were refused with
+#: HSA_STATUS_ERROR_OUT_OF_RESOURCES, which aborts the queue. A build whose kernels need more than
+#: the size shown to run fails here, on the CPU.
+MAX_SCRATCH_PER_LANE = 29568
 
 _REMARK = re.compile(r"remark:\s+(?:Function Name: (\S+)|([A-Za-z ]+?)(?: \[[^\]]*\])?: "
                      r"(\S+)) \[-Rpass-analysis=kernel-resource-usage\]")

@@ -283,26 +283,26 @@ __device__ inline double lu_solve_group(const T (&row)[n], int perm_l, double b,
   return l < n ? double(x) : 0.0;
 }
 
-// Equilibration of the FP32 Newton matrix: component i is scaled by d_i, the power of two at or
-// below its error weight ewt_i (clamped to 2^-60..2^60), and the factors are those of D M D^-1.
-// Powers of two make the scaling exact in FP64, so only the FP32 cast of D M D^-1 rounds. In the
-// error-weighted variables the Newton convergence test measures, the scaled matrix is well
-// conditioned where M itself (rho Y_k ~ 1e-4 beside T ~ 1e3) is not.
-__device__ inline double pow2_scale(double w) {
+// Equilibration of the FP32 Newton matrix: component i is scaled by d_i = 2^k_i, the power of
+// two at or below its error weight ewt_i (k_i clamped to -60..60), and the factors are those of
+// D M D^-1. Powers of two make the scaling exact (one ldexp per element), so only the FP32 cast
+// of D M D^-1 rounds. In the error-weighted variables the Newton convergence test measures, the
+// scaled matrix is well conditioned where M itself (rho Y_k ~ 1e-4 beside T ~ 1e3) is not.
+__device__ inline int scale_exp(double w) {
   const int k = ilogb(w);
-  return ldexp(1.0, k < -60 ? -60 : (k > 60 ? 60 : k));
+  return k < -60 ? -60 : (k > 60 ? 60 : k);
 }
 
 // x = (LU)^{-1} P b with factors from lu_rows (AoS block `lu`, inverted diagonal). b in/out (FP64),
 // arithmetic in T. `gb` is this lane's LDS scratch (stride 64) used to gather b by the permutation.
-// With `dsc` (the equilibration of an FP32 matrix, one power of two per component) the factors
-// are those of D M D^-1: b is scaled by D on the way in and x by D^-1 on the way out.
+// With `dexp` (the equilibration exponents k_i of an FP32 matrix) the factors are those of
+// D M D^-1: b is scaled by D on the way in and x by D^-1 on the way out.
 template <int n, typename T>
 __device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b, double* gb,
-                                     const double* dsc = nullptr) {
-  if (dsc) {
+                                     const int32_t* dexp = nullptr) {
+  if (dexp) {
 #pragma unroll
-    for (int i = 0; i < n; ++i) gb[i * 64] = b[i] * dsc[i];
+    for (int i = 0; i < n; ++i) gb[i * 64] = ldexp(b[i], dexp[i]);
   } else {
 #pragma unroll
     for (int i = 0; i < n; ++i) gb[i * 64] = b[i];
@@ -328,9 +328,9 @@ __device__ inline void lu_solve_perm(const T* lu, const int32_t* perm, double* b
     x[i] = s * lu[i * n + i];
     PELEC_MEMORY_FENCE();
   }
-  if (dsc) {
+  if (dexp) {
 #pragma unroll
-    for (int i = 0; i < n; ++i) b[i] = double(x[i]) / dsc[i];
+    for (int i = 0; i < n; ++i) b[i] = ldexp(double(x[i]), -dexp[i]);
   } else {
 #pragma unroll
     for (int i = 0; i < n; ++i) b[i] = double(x[i]);

@@ -1,5 +1,6 @@
-// Cooperative form of the constant-volume rho*e reactor model: one group of G lanes (G = 32, two
-// cells per 64-lane wave) evaluates one cell's RHS and exact Jacobian.
+// Cooperative form of the constant-volume rho*e reactor model: one group of G lanes evaluates one
+// cell's RHS and exact Jacobian (G = 32, two cells per 64-lane wave, for n <= 32; G = 64, one cell
+// per wave, for the 53-species mechanisms).
 //
 // Same mathematics as CellModel (cell_model.h: ReactorCvodeUtils / ReactorCvodeJacobian /
 // ReactorUtils of the reference, CMake/BuildPelePhysicsLib.cmake:132-141), laid out for lanes:
@@ -52,6 +53,7 @@ __device__ inline X after(X x, D dep) {
 
 template <class Mech, int G = 32>
 struct GroupModel {
+  static constexpr int GS = G;  // lanes per cell
   using Grp = Group<G>;
   using Model = CellModel<Mech>;
   static constexpr int N = Mech::NSP;

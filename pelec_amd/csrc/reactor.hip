@@ -48,10 +48,11 @@ struct Shape {
   // one lane's latency, which the group form divides among 32 lanes. 0: off.
   int tail_slots = 0;
   int bpc_tail = 0;
-  // FP64 refinement steps after each FP32-LU solve (prc.lu_refine). Two make the Newton
-  // corrections FP64-equivalent on every drm19 64^3 cell; with one, a marginal step decision in
-  // one cell of 262144 flipped (4.6e-3 K off the oracle).
-  int refine = 2;
+  // FP64 refinement steps after each FP32-LU solve (prc.lu_refine). With the equilibrated factors
+  // one step makes the Newton corrections FP64-equivalent on every drm19 64^3 cell (each within
+  // 7.3e-5 K of the oracle, as with FP64 factors); with none, 1333 of its 262144 cells took
+  // other step decisions and ended up to 0.056 K off.
+  int refine = 1;
   bool operator==(const Shape& o) const {
     return iters == o.iters && coop == o.coop && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
            bpc_lu == o.bpc_lu && compact_pct == o.compact_pct && tail_slots == o.tail_slots &&
@@ -67,12 +68,18 @@ struct PhaseKernels {
 };
 template <typename LUT, int NREF>
 PhaseKernels phase_kernels_r(const Shape& sh, bool tail) {
-  PhaseKernels k{&k_newton<LUT, NREF>, &k_jac};
-  if constexpr (COOP) {
-    if ((sh.coop & 1) || tail) k.newton = &k_newton_g<LUT, 64, NREF>;
-    if (sh.coop & 2) k.jac = &k_jac_g<64>;
+  if constexpr (COOP64) {
+    // 53-species mechanisms: one cell per 64-lane wave, the only Newton and Jacobian kernels built
+    using GM64 = GroupModel<Mech, 64>;
+    return PhaseKernels{&k_newton_g<LUT, 64, NREF, GM64>, &k_jac_g<64, GM64>};
+  } else {
+    PhaseKernels k{&k_newton<LUT, NREF>, &k_jac<>};
+    if constexpr (COOP) {
+      if ((sh.coop & 1) || tail) k.newton = &k_newton_g<LUT, 64, NREF>;
+      if (sh.coop & 2) k.jac = &k_jac_g<64>;
+    }
+    return k;
   }
-  return k;
 }
 template <typename LUT>
 PhaseKernels phase_kernels(const Shape& sh, bool tail = false) {
@@ -222,7 +229,7 @@ size_t workspace_layout(Args& L, char* base, int64_t C, bool lu_fp32, bool has_s
   L.J = reinterpret_cast<double*>(take(sizeof(double) * NV * NV * C));
   L.LU = take(lub * NV * NV * C);
   L.perm = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * NV * C));
-  L.dsc = lu_fp32 ? reinterpret_cast<double*>(take(sizeof(double) * NV * C)) : nullptr;
+  L.dexp = lu_fp32 ? reinterpret_cast<int32_t*>(take(sizeof(int32_t) * NV * C)) : nullptr;
   for (int b = 0; b < NLISTS; ++b) L.list[b] = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
   L.remap = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
   L.ring = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
@@ -544,7 +551,7 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   a.J = L.J;
   a.LU = L.LU;
   a.perm = L.perm;
-  a.dsc = L.dsc;
+  a.dexp = L.dexp;
   for (int b = 0; b < NLISTS; ++b) a.list[b] = L.list[b];
   a.remap = L.remap;
   a.ring = L.ring;
@@ -1256,6 +1263,9 @@ int prc_model_eval(int64_t nc, double t, const double* y, const double* rhoe0,
     if constexpr (COOP)
       hipLaunchKernelGGL(k_model_group<64>, dim3((unsigned)((nc + 1) / 2)), dim3(64), 0, s, nc, t,
                          y, rhoe0, rhoesrc, rYsrc, ydot, J);
+    else if constexpr (COOP64)
+      hipLaunchKernelGGL((k_model_group<64, GroupModel<Mech, 64>>), dim3((unsigned)nc), dim3(64), 0,
+                         s, nc, t, y, rhoe0, rhoesrc, rYsrc, ydot, J);
     else
       return PRC_EINVAL;  // this mechanism has no cooperative kernels
   } else

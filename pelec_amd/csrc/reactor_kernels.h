@@ -68,7 +68,7 @@ struct Args {
   double* J;       // per matrix block n*n (FP64)
   void* LU;        // per matrix block n*n (LUT)
   int32_t* perm;   // per matrix block n
-  double* dsc;     // per matrix block n: equilibration of FP32 factors (pow2_scale), else unused
+  int32_t* dexp;   // per matrix block n: equilibration exponents of FP32 factors (scale_exp)
   int32_t* list[NLISTS];
   int32_t* remap;
   int32_t* ring;   // C: free matrix blocks (tail counter CT_FTAIL, head kept by the host)
@@ -346,7 +346,9 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap, int64
   wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, unused, mblk);
 }
 
-// Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision)
+// Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision). A template so
+// that mechanisms whose Jacobian runs in the cooperative kernel do not build it.
+template <int UNUSED = 0>
 __global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par) {
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
@@ -413,17 +415,17 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
     }
     LUT row[NV];
     constexpr bool EQ = sizeof(LUT) == sizeof(float);  // FP32 factors are equilibrated
-    double dr = 1.0;
+    int kr = 0;
     if (valid && r < NV) {
       const double* Jr = a.J + g * (int64_t)(NV * NV) + r * NV;
       if constexpr (EQ) {
-        // D (I - gamma J) D^-1 with d_c the power of two of the cell's error weight ewt_c
+        // D (I - gamma J) D^-1 with d_c = 2^k_c the power of two of the cell's error weight ewt_c
         const double* ew = st.ewt + s;
-        dr = pow2_scale(ew[(int64_t)r * a.C]);
+        kr = scale_exp(ew[(int64_t)r * a.C]);
 #pragma unroll
         for (int c = 0; c < NV; ++c) {
           const double m = (c == r ? 1.0 : 0.0) - gamma * Jr[c];
-          row[c] = LUT(m * dr / pow2_scale(ew[(int64_t)c * a.C]));
+          row[c] = LUT(ldexp(m, kr - scale_exp(ew[(int64_t)c * a.C])));
         }
       } else {
 #pragma unroll
@@ -445,7 +447,7 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
 #pragma unroll
       for (int c = 0; c < NV; ++c) LUg[c] = row[c];
       a.perm[g * NV + pos] = r;
-      if constexpr (EQ) a.dsc[g * NV + r] = dr;
+      if constexpr (EQ) a.dexp[g * NV + r] = kr;
     }
     if (valid && leader) {
       after_setup_slot(st.sc, a.C, s);
@@ -513,19 +515,19 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
         const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV);
         const int32_t* pm = a.perm + mb * NV;
         const double* Jg = a.J + mb * (int64_t)(NV * NV);
-        const double* dsc = a.dsc + mb * NV;
+        const int32_t* dexp = a.dexp + mb * NV;
         const double gammap = S.gammap;
         double* gbl = gb + lane;
         auto solve = [&](double* b) {
           if constexpr (sizeof(LUT) == sizeof(double)) {
             lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
           } else if constexpr (NREF == 0) {
-            lu_solve_perm<NV, LUT>(lu, pm, b, gbl, dsc);
+            lu_solve_perm<NV, LUT>(lu, pm, b, gbl, dexp);
           } else {
             double x[NV], res[NV];
 #pragma unroll
             for (int e = 0; e < NV; ++e) x[e] = b[e];
-            lu_solve_perm<NV, LUT>(lu, pm, x, gbl, dsc);
+            lu_solve_perm<NV, LUT>(lu, pm, x, gbl, dexp);
 #pragma unroll
             for (int it = 0; it < NREF; ++it) {
 #pragma unroll
@@ -536,7 +538,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
                 res[e] = b[e] - (x[e] - gammap * jx);
                 PELEC_MEMORY_FENCE();
               }
-              lu_solve_perm<NV, LUT>(lu, pm, res, gbl, dsc);
+              lu_solve_perm<NV, LUT>(lu, pm, res, gbl, dexp);
 #pragma unroll
               for (int e = 0; e < NV; ++e) x[e] += res[e];
             }
@@ -613,16 +615,23 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
 }
 
 // ------------------------------------------------------------------ cooperative kernels
-// One cell per 32-lane group (two per wave): lane k holds state component k, the species and
-// reaction work of the RHS and Jacobian is spread over the group (group_model.h), and the CVODE
-// step logic runs on the group accessor (component k on lane k, the integrator scalars replicated,
-// norms as group reductions). Same phases, lists, slot storage and counters as the one-cell-per-lane
-// kernels, which they replace one for one (prc.cooperative).
+// One cell per G-lane group: lane k holds state component k, the species and reaction work of the
+// RHS and Jacobian is spread over the group (group_model.h), and the CVODE step logic runs on the
+// group accessor (component k on lane k, the integrator scalars replicated, norms as group
+// reductions). Same phases, lists, slot storage and counters as the one-cell-per-lane kernels,
+// which they replace one for one.
+//   n <= 32: G = 32, two cells per wave, chosen by prc.cooperative / the tail graph;
+//   32 < n <= 64 (the 53-species mechanisms): G = 64, one cell per wave, always -- their
+//   one-cell-per-lane Newton and Jacobian kernels would hold 54-component vectors and factor rows
+//   per lane (registers spill to scratch, and the unrolled n^2 solves took the compiler tens of
+//   minutes), so those are not built.
 constexpr int GRP = 32;
 using GModel = GroupModel<Mech, GRP>;
 // The cooperative kernels exist for mechanisms whose data tables were generated and whose state
-// fits one lane per component (n <= 32); for others only the one-cell-per-lane kernels are built.
+// fits one lane per component of a 32- or 64-lane group; for others only the one-cell-per-lane
+// kernels are built.
 constexpr bool COOP = Mech::T_OK && NV <= GRP;
+constexpr bool COOP64 = Mech::T_OK && NV > GRP && NV <= 64;
 
 // store_cell for a group: lane k < N writes rhoY_k, T(e_end, Y) by the group's T(e).
 template <class GM = GModel>
@@ -653,15 +662,16 @@ __device__ inline void store_cell_g(const Args& a, int64_t g, int status, const 
 // Jacobians of the set-up list entries that need one, one cell per group (k_jac's work).
 template <int BS, class GM = GModel>
 __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int par) {
-  constexpr int CPB = BS / GRP;
+  constexpr int G = GM::GS;
+  constexpr int CPB = BS / G;
   __shared__ double lds[CPB * GM::JAC_WORDS];
   const Args& a = *ap;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
-  const int l = threadIdx.x & (GRP - 1);
-  double* sh = lds + (threadIdx.x / GRP) * GM::JAC_WORDS;
+  const int l = threadIdx.x & (G - 1);
+  double* sh = lds + (threadIdx.x / G) * GM::JAC_WORDS;
   for (int64_t base = (int64_t)blockIdx.x * CPB; base < cnt; base += (int64_t)gridDim.x * CPB) {
-    const int64_t idx = base + threadIdx.x / GRP;
+    const int64_t idx = base + threadIdx.x / G;
     if (idx >= cnt) continue;
     const int64_t s = a.list[par][idx];
     bool need;
@@ -692,7 +702,8 @@ __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int p
 // One Newton correction per Newton-phase slot, one cell per group (k_newton's work).
 template <typename LUT, int BS, int NREF, class GM = GModel>
 __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, int par) {
-  constexpr int CPB = BS / GRP;
+  constexpr int G = GM::GS;
+  constexpr int CPB = BS / G;
   __shared__ double lds[CPB * GM::RHS_WORDS];
   // the integrator scalars of each cell live in LDS, one copy per group (in registers they would
   // be replicated on every lane of the group and more than halve the occupancy)
@@ -701,24 +712,24 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
-  const int l = threadIdx.x & (GRP - 1);
-  double* sh = lds + (threadIdx.x / GRP) * GM::RHS_WORDS;
+  const int l = threadIdx.x & (G - 1);
+  double* sh = lds + (threadIdx.x / G) * GM::RHS_WORDS;
   for (int64_t base = (int64_t)blockIdx.x * CPB; base < nact; base += (int64_t)gridDim.x * CPB) {
-    const int64_t s = base + threadIdx.x / GRP;
+    const int64_t s = base + threadIdx.x / G;
     const int ph = s < nact ? st.phase[s] : PH_DONE;
     const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
     const bool lufail = ph == PH_LUFAIL0 + list_prev(par);  // as in k_newton
     bool need_setup = false, done = false;
     if (act || lufail) {
-      const GrpV<NV, GRP> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
+      const GrpV<NV, G> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
       const int64_t g = st.cell[s];
       const int64_t mb = st.mb[s];
       const double rhoe0 = st.rhoe[s], rhoesrc = st.rhoe[a.C + s];
       const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
-      BdfScalars& S = scl[threadIdx.x / GRP];
+      BdfScalars& S = scl[threadIdx.x / G];
       int r = 3;
       if (lufail) {
-        load_scalars_group<GRP>(S, st.sc, a.C, s, l);
+        load_scalars_group<G>(S, st.sc, a.C, s, l);
       } else {
         constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
         constexpr int W_M = offsetof(BdfScalars, m) / 8;
@@ -733,30 +744,30 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
           const double yl = l < NV ? v.Z(0, l) + (m0 > 0 ? v.A(l) : 0.0) : 0.0;
           ftemp.x = GM::rhs(tn, yl, l, rhoe0, rhoesrc, srcl, sh);
         }
-        load_scalars_group<GRP>(S, st.sc, a.C, s, l);
+        load_scalars_group<G>(S, st.sc, a.C, s, l);
         S.nfe++;
         if (S.m == 0 && l < NV) v.A(l) = 0.0;
         const double gammap = S.gammap;
         auto solve = [&](LaneVal& b) {
           LUT row[NV];
           int pl = l;
-          double dl = 1.0;  // this lane's component of the equilibration (FP32 factors)
+          int el = 0;  // this lane's equilibration exponent (FP32 factors)
           if (l < NV) {
             const LUT* lr = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV) + l * NV;
 #pragma unroll
             for (int c = 0; c < NV; ++c) row[c] = lr[c];
             pl = a.perm[mb * NV + l];
-            if constexpr (sizeof(LUT) != sizeof(double)) dl = a.dsc[mb * NV + l];
+            if constexpr (sizeof(LUT) != sizeof(double)) el = a.dexp[mb * NV + l];
           } else {
 #pragma unroll
             for (int c = 0; c < NV; ++c) row[c] = LUT(0);
           }
           if constexpr (sizeof(LUT) == sizeof(double)) {
-            b.x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x, l);
+            b.x = lu_solve_group<NV, LUT, G>(row, pl, b.x, l);
           } else {
             // FP32 factors of D M D^-1, then NREF FP64 refinement steps against the FP64
             // Jacobian (k_newton's)
-            double x = lu_solve_group<NV, LUT, GRP>(row, pl, b.x * dl, l) / dl;
+            double x = ldexp(lu_solve_group<NV, LUT, G>(row, pl, ldexp(b.x, el), l), -el);
             const double* Jr = a.J + mb * (int64_t)(NV * NV) + (l < NV ? l : 0) * NV;
 #pragma unroll
             for (int it = 0; it < NREF; ++it) {
@@ -767,7 +778,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
                 if (l < NV) jx += Jr[c] * xc;
               }
               const double rr = l < NV ? b.x - (x - gammap * jx) : 0.0;
-              x += lu_solve_group<NV, LUT, GRP>(row, pl, rr * dl, l) / dl;
+              x += ldexp(lu_solve_group<NV, LUT, G>(row, pl, ldexp(rr, el), l), -el);
             }
             b.x = x;
           }
@@ -830,7 +841,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
         else if (ph != PH_NEWTON)
           st.phase[s] = PH_NEWTON;
       }
-      store_scalars_group<GRP>(S, st.sc, a.C, s, l);
+      store_scalars_group<G>(S, st.sc, a.C, s, l);
     }
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup && l == 0,
                 (int32_t)s);
@@ -1306,17 +1317,18 @@ __global__ void __launch_bounds__(64) k_model_lane(int64_t nc, double t, const d
   if (J) Model::template jac<double>(t, yy, rhoe0[c], r1, src ? src + c : nullptr, nc, J + c * (int64_t)(NV * NV));
 }
 
-// ... or one cell per 32-lane group (GroupModel), two per block.
+// ... or one cell per G-lane group (GroupModel), BS / G per block.
 template <int BS, class GM = GModel>
 __global__ void __launch_bounds__(BS) k_model_group(int64_t nc, double t, const double* y,
                                                      const double* rhoe0, const double* rhoesrc,
                                                      const double* src, double* ydot, double* J) {
-  static_assert(BS == 2 * GRP, "two cells per block");
-  __shared__ double lds[2 * GM::JAC_WORDS];
-  const int64_t c = (int64_t)blockIdx.x * 2 + threadIdx.x / GRP;
-  const int l = threadIdx.x & (GRP - 1);
+  constexpr int G = GM::GS;
+  constexpr int CPB = BS / G;
+  __shared__ double lds[CPB * GM::JAC_WORDS];
+  const int64_t c = (int64_t)blockIdx.x * CPB + threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
   if (c >= nc) return;
-  double* sh = lds + (threadIdx.x / GRP) * GM::JAC_WORDS;
+  double* sh = lds + (threadIdx.x / G) * GM::JAC_WORDS;
   const double yl = l < NV ? y[l * nc + c] : 0.0;
   const double srcl = (src && l < NSP) ? src[l * nc + c] : 0.0;
   const double r1 = rhoesrc ? rhoesrc[c] : 0.0;

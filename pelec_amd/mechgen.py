@@ -492,12 +492,16 @@ class Gen:
         for r in self.m.reactions:
             if r.A <= 0.0:
                 return f"{r.equation}: A <= 0"
-            for side in (r.reactants, r.products):
+            # the product side enters the rate only through the reverse rate: an irreversible
+            # reaction may have any products (its product slots stay empty)
+            for side in ((r.reactants, r.products) if r.reversible else (r.reactants,)):
                 if len(side) > 3 or any(v != int(v) or not 1 <= v <= 3 for v in side.values()):
                     return f"{r.equation}: more than 3 species or a non-integer order on one side"
             d = sum(r.products.values()) - sum(r.reactants.values())
-            if d != int(d) or abs(d) > 2:
+            if r.reversible and (d != int(d) or abs(d) > 2):
                 return f"{r.equation}: |dnu| > 2"
+            if len(self.nu_of(r)) > 6:
+                return f"{r.equation}: more than 6 species change"
         if len(self.effsets) > 32:
             return "more than 32 efficiency sets"
         return None
@@ -593,12 +597,12 @@ class Gen:
                 troe.append([0.0] * 8)
             eff.append(self.effsets[tuple(r.efficiencies)] if r.efficiencies is not None else -1)
             for side, si, ni in ((r.reactants, rsp, rnu), (r.products, psp, pnu)):
-                items = sorted(side.items())
+                items = sorted(side.items()) if (side is r.reactants or r.reversible) else []
                 if len(items) > 3 or any(v != int(v) or not 1 <= v <= 3 for _, v in items):
                     raise ValueError(f"{r.equation}: table form needs <= 3 species, nu in 1..3")
                 si.append([k for k, _ in items] + [-1] * (3 - len(items)))
                 ni.append([int(v) for _, v in items] + [0] * (3 - len(items)))
-            d = sum(r.products.values()) - sum(r.reactants.values())
+            d = sum(r.products.values()) - sum(r.reactants.values()) if r.reversible else 0
             if d != int(d) or abs(d) > 2:
                 raise ValueError(f"{r.equation}: table form needs |dnu| <= 2")
             dnu.append(int(d))

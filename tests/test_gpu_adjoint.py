@@ -5,7 +5,8 @@ held fixed. The oracle replays the device's recorded tape (the same Nordsieck op
 corrector solved to convergence) and provides (1) the end state, (2) forward sensitivities by its
 tangent-linear replay along sampled directions and (3) central finite differences (h = 1e-6) of
 the replay. Acceptance: adjoint within 1e-5 relative of the finite differences, and of the forward
-sensitivities. Workload: drm19 cells of the adjoint configuration (dt = 1e-4 s, through ignition).
+sensitivities. Workload: 32 drm19 cells drawn evenly from the 16384 of the adjoint configuration
+(dt = 1e-4 s, through ignition); 64 sampled log-A parameters.
 """
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -33,11 +34,16 @@ def to_dev(a, dtype=torch.float64):
     return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
 
 
+#: cells of the adjoint workload checked here: 32 drawn evenly from its 16384
+CELLS = np.arange(0, 16384, 512)
+
+
 @pytest.fixture(scope="module")
 def adjoint_case(oracle_lib):
     W = inputs.WORKLOADS["drm19_adj"]
-    n = 4
-    d = inputs.make_inputs("drm19_adj", n)
+    full = inputs.make_inputs("drm19_adj")
+    d = {"rY": full["rY"][:, CELLS], "T": full["T"][CELLS], "rE": full["rE"][CELLS]}
+    n = CELLS.size
     m = M.load("drm19")
     rng = np.random.default_rng(404)
     rho = d["rY"].sum(0)
@@ -78,10 +84,11 @@ def test_adjoint_forward_matches_replay(adjoint_case):
         assert np.abs(yend[:-1] - res["rY"][:, c]).max() <= 1e-8 * yend[:-1].sum()
 
 
-def test_adjoint_vs_forward_sensitivities(adjoint_case):
-    """Adjoint against the tangent-linear replay: 64 sampled reactions (summed over cells), every
-    initial rhoY component of cell 0 and rho*e0 of every cell."""
+def test_adjoint_vs_forward_sensitivities(adjoint_case, oracle_lib):
+    """Adjoint against the tangent-linear replay: 64 sampled reactions (summed over the 32 cells),
+    every initial rhoY component of the first four cells and rho*e0 of every cell."""
     W, d, m, ct_rY, ct_T, res, om = adjoint_case
+    om_fma = oracle_lib.OracleMech(m, variant="fma")
     n = d["T"].shape[0]
     R = m.nrxn
     rng = np.random.default_rng(7)
@@ -100,31 +107,43 @@ def test_adjoint_vs_forward_sensitivities(adjoint_case):
     err = np.abs(g_logA[js] - tl)
     note(f"  logA: max|tangent| {scale:.3e}, max rel err {(err / np.maximum(np.abs(tl), 1e-300)).max():.2e}")
     assert (err <= REL * np.abs(tl) + 1e-9 * scale).all(), (js[err > REL * np.abs(tl) + 1e-9 * scale])
-    # initial rhoY of cell 0: species present at t = 0 to 1e-5; species that start at exactly zero
-    # and relax within the first step (CH2(S), H, CH2, HCO: eigenvalues to -1.7e8 /s) have their
-    # initial-value sensitivity routed through Newton matrices with cond(I - gamma J) up to 1e20,
-    # and three exact-arithmetic-equivalent evaluations (adjoint, tangent, explicit product of the
-    # linearised operations) spread by up to 1e-3 there: checked to 2e-3
-    present = d["rY"][:, 0] > 0.0
-    for k in range(m.nsp):
-        dy = np.zeros(m.nsp + 1)
-        dy[k] = 1.0
-        t = _loss(om, res, d, 0, ct_rY, ct_T, W.dt, tangent=(dy, 0.0, None))[1]
-        tol = REL if present[k] else 2e-3
-        assert abs(res["g_rY0"][k, 0] - t) <= tol * abs(t) + 1e-300, (k, res["g_rY0"][k, 0], t)
+    # initial rhoY of the first four cells, every component. Species present at t = 0: within
+    # 1e-5 relative. Species that start at exactly zero and relax within the first step (CH2(S),
+    # H, CH2, HCO: eigenvalues to -1.7e8 /s) have sensitivities routed through Newton matrices with
+    # cond(I - gamma J) up to 1e20, so their value is round-off-bound: two FP64 builds of the
+    # oracle's tangent (with and without FMA contraction) on the same tape already differ. They
+    # are held to 1e-5 of the gradient's largest component plus 8 times that build-to-build
+    # difference.
+    worst = 0.0
+    for c in range(4):
+        tk = np.empty(m.nsp)
+        tf = np.empty(m.nsp)
+        for k in range(m.nsp):
+            dy = np.zeros(m.nsp + 1)
+            dy[k] = 1.0
+            tk[k] = _loss(om, res, d, c, ct_rY, ct_T, W.dt, tangent=(dy, 0.0, None))[1]
+            tf[k] = _loss(om_fma, res, d, c, ct_rY, ct_T, W.dt, tangent=(dy, 0.0, None))[1]
+        err = np.abs(res["g_rY0"][:, c] - tk)
+        present = d["rY"][:, c] > 0.0
+        lim = np.where(present, REL * np.abs(tk), REL * np.abs(tk).max() + 8.0 * np.abs(tk - tf))
+        worst = max(worst, (err / np.abs(tk).max()).max())
+        assert (err <= lim).all(), (c, err, lim)
+    note(f"  initial rhoY: max error relative to the gradient norm {worst:.2e}")
     for c in range(n):
         t = _loss(om, res, d, c, ct_rY, ct_T, W.dt, tangent=(np.zeros(m.nsp + 1), 1.0, None))[1]
         assert abs(res["g_rE0"][c] - t) <= REL * abs(t), (c, res["g_rE0"][c], t)
 
 
 def test_adjoint_vs_central_differences(adjoint_case):
-    """Central differences (h = 1e-6) of the replayed map on the eight reactions with the largest
-    sensitivities, and on rho*e0 of cell 0: within 1e-5 relative."""
+    """Central differences (h = 1e-6) of the replayed map, summed over the 32 cells, on 64 sampled
+    reactions: within 1e-5 relative plus the differences' own round-off, 1e-8 of the summed
+    |loss| (an FP64 difference quotient carries about eps |L| / h = 2e-10 |L|); and on rho*e0
+    of cell 0."""
     W, d, m, ct_rY, ct_T, res, om = adjoint_case
     n = d["T"].shape[0]
     R = m.nrxn
     g = res["g_logA"].sum(1)
-    js = np.argsort(-np.abs(g))[:8]
+    js = np.sort(np.random.default_rng(64).choice(R, size=64, replace=False))
     h = 1e-6
 
     def fd(j):
@@ -136,11 +155,16 @@ def test_adjoint_vs_central_differences(adjoint_case):
                     - _loss(om, res, d, c, ct_rY, ct_T, W.dt, logA=-lp)) / (2 * h)
         return tot
 
-    with ThreadPoolExecutor(8) as ex:
+    with ThreadPoolExecutor(16) as ex:
         fds = np.array(list(ex.map(fd, js)))
-    rel = np.abs(g[js] - fds) / np.abs(fds)
-    note(f"  FD logA rel err: {rel.max():.2e}")
-    assert (rel <= REL).all(), list(zip(js, g[js], fds))
+    Labs = sum(abs(_loss(om, res, d, c, ct_rY, ct_T, W.dt)) for c in range(n))
+    floor = 1e-8 * Labs
+    err = np.abs(g[js] - fds)
+    big = np.abs(fds) > 1e3 * floor
+    note(f"  FD logA ({js.size} reactions x {n} cells, sum |L| {Labs:.3e}): max rel err "
+         f"{(err[big] / np.abs(fds[big])).max():.2e} on the {int(big.sum())} above 1e3 x the FD "
+         f"round-off floor, max err {err.max():.2e} against the floor {floor:.2e}")
+    assert (err <= REL * np.abs(fds) + floor).all(), list(zip(js, g[js], fds))
     he = 1e-6 * d["rE"][0]
     fe = (_loss(om, res, d, 0, ct_rY, ct_T, W.dt, rE0=d["rE"][0] + he)
           - _loss(om, res, d, 0, ct_rY, ct_T, W.dt, rE0=d["rE"][0] - he)) / (2 * he)
