@@ -32,6 +32,7 @@
 #ifndef PELEC_AMD_H
 #define PELEC_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -166,6 +167,25 @@ typedef struct prc_box_args {
 } prc_box_args;
 int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, double* dt_react,
                     double* time, void* stream);
+
+/* No-allocation variant of prc_react_flat. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results as prc_react_flat with the
+ * batched form (ReactorCvode; ReactorNull only advances *time; ReactorRK64 is rejected with
+ * PRC_EINVAL), whatever prc.form says. It calls no hipMalloc/hipFree: the argument block, the
+ * work-queue counters and the batched workspace live in a static device array of
+ * PRC_NOALLOC_SCRATCH_BYTES per GPU (held by the library until the process ends, located at
+ * prc_init), and the work queue gets as many slots as that array holds (drm19 about 13,000,
+ * GRI-Mech 3.0 about 3,000), refilled from the call's cells, so any ncells up to INT32_MAX is
+ * accepted. PRC_ENOMEM if not even 64 slots fit; PRC_ESTATE if the handle was initialised on
+ * another device. Iterations are enqueued directly (no graph is instantiated). Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU -- from any handle of this
+ * mechanism's library -- race, and must be serialised by the caller. Blocks like prc_react_flat;
+ * prc_last_failures reports the call's failures. */
+#define PRC_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int prc_react_flat_noalloc(prc_reactor* r, double* rY, const double* rYsrc, double* T, double* rE,
+                           const double* rEsrc, double* FC, const int32_t* mask, int64_t ncells,
+                           double* dt_react, double* time, void* stream);
 
 /* ---- PeleC-side passes around the reactor ----------------------------------------------- */
 /* PeleC::react_state for one box (Source/React.cpp:29-362, the per-MFIter body), fused: one pack

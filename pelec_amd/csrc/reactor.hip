@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -404,7 +405,7 @@ int eager_accumulate(prc_reactor* r) {
 }
 
 // The batched form of one call. `dargs` / `dctr` are the device argument block and counters
-// (the handle's, or a caller-provided block). With `graphs`, iterations run as
+// (the handle's, or the no-allocation entry point's scratch). With `graphs`, iterations run as
 // replays of the handle's captured graphs, which read the handle's argument block; without, they
 // are enqueued directly (no graph is instantiated).
 template <typename LUT>
@@ -618,6 +619,37 @@ void release(prc_reactor* r) {
   r->initialised = false;
 }
 
+// ---- the no-allocation entry point's scratch (prc_react_flat_noalloc) ------------------------
+// One array per GPU the library runs on, held until the process ends: the argument block, the
+// work-queue counters and the batched workspace of prc_react_flat_noalloc. In host code `scratch`
+// names a host-side stand-in; its device address on each GPU is looked up once per device at
+// prc_init (hipGetSymbolAddress) and only that address is passed to kernels or copies.
+constexpr size_t NOALLOC_SCRATCH_BYTES = PRC_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* g_scratch[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// Device address of this library's scratch on the current device (set-up, under the lock).
+int scratch_setup() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return PRC_EHIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return PRC_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PRC_EHIP;
+  g_scratch[dev] = ptr;
+  return PRC_OK;
+}
+
+void* scratch_ptr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -663,7 +695,8 @@ int prc_init(prc_reactor* r, int reactor_type, int ncells) {
     if (hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&r->cap_stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
+        scratch_setup() != PRC_OK) {
       release(r);
       return PRC_EHIP;
     }
@@ -874,6 +907,70 @@ int prc_react_flat(prc_reactor* r, double* rY, const double* rYsrc, double* T, d
   m.ncomp = 1;
   return prc_react_box(r, lo, hi, view(rY, NSP), view(rYsrc, NSP), view(T, 1), view(rE, 1),
                        view(rEsrc, 1), view(FC, 1), m, dt_react, time, stream);
+}
+
+int prc_react_flat_noalloc(prc_reactor* r, double* rY, const double* rYsrc, double* T, double* rE,
+                           const double* rEsrc, double* FC, const int32_t* mask, int64_t ncells,
+                           double* dt_react, double* time, void* stream) {
+  int e = ensure_init(r);
+  if (e) return e;
+  if (ncells < 0 || ncells > INT32_MAX || !dt_react || !time) return PRC_EINVAL;
+  if (r->integrator == 1) return PRC_EINVAL;  // ReactorRK64 has no batched form
+  if (ncells == 0 || r->integrator == 2) {
+    *time += *dt_react;
+    return PRC_OK;
+  }
+  if (!rY || !T || !rE) return PRC_EINVAL;
+  char* base = static_cast<char*>(scratch_ptr());
+  if (!base) return PRC_ESTATE;  // prc_init ran on another device
+  hipStream_t s = (hipStream_t)stream;
+  // scratch: [argument block][counters][batched workspace of as many slots as fit]
+  const size_t o_ctr = align_up(sizeof(Args));
+  const size_t o_ws = o_ctr + align_up(CT_COUNT * sizeof(unsigned long long));
+  const bool has_src = rYsrc != nullptr;
+  Args a{};
+  const size_t per64 = workspace_layout(a, nullptr, 64, r->lu_fp32, has_src);
+  if (o_ws + per64 > NOALLOC_SCRATCH_BYTES) return PRC_ENOMEM;  // not even 64 slots fit
+  int64_t C = 64;
+  while (o_ws + workspace_layout(a, nullptr, C + 64, r->lu_fp32, has_src) <= NOALLOC_SCRATCH_BYTES &&
+         C < ncells)
+    C += 64;
+  workspace_layout(a, base + o_ws, C, r->lu_fp32, has_src);
+  Args* dargs = reinterpret_cast<Args*>(base);
+  unsigned long long* dctr = reinterpret_cast<unsigned long long*>(base + o_ctr);
+  fill_params(r, a, dctr);
+  auto view = [&](const double* p) {
+    A4 v{};
+    v.p = const_cast<double*>(p);
+    v.js = v.ks = v.ns = ncells;
+    return v;
+  };
+  a.box.lo0 = a.box.lo1 = a.box.lo2 = 0;
+  a.box.len0 = (int)ncells;
+  a.box.len1 = 1;
+  a.box.ncells = ncells;
+  a.box.first = 0;
+  a.box.rY = view(rY);
+  a.box.rYsrc = view(rYsrc);
+  a.box.T = view(T);
+  a.box.rE = view(rE);
+  a.box.rEsrc = view(rEsrc);
+  a.box.FC = view(FC);
+  a.box.mask = A4I{const_cast<int32_t*>(mask), ncells, ncells, ncells, 0, 0, 0};
+  a.boxes = nullptr;
+  a.nboxes = 1;
+  a.ncells = ncells;
+  a.dt = *dt_react;
+  a.has_src = has_src ? 1 : 0;
+  a.test_lu_fail = r->test_lu_fail;
+  e = r->lu_fp32 ? run_batched<float>(r, a, s, dargs, dctr, false)
+                 : run_batched<double>(r, a, s, dargs, dctr, false);
+  if (e) return e;
+  // prc_last_failures reads the handle's counters
+  HIP_OK(hipMemcpyAsync(r->d_ctr, dctr, CT_COUNT * sizeof(unsigned long long),
+                        hipMemcpyDeviceToDevice, s));
+  *time += *dt_react;
+  return PRC_OK;
 }
 
 int prc_react_boxes(prc_reactor* r, int nboxes, const prc_box_args* boxes, double* dt_react,

@@ -74,6 +74,9 @@ _SIGS = {
     "prc_react_flat": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [_i64,
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_void_p]),
+    "prc_react_flat_noalloc": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 7 + [_i64,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.c_void_p]),
     "prc_react_boxes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BoxArgs),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.c_void_p]),
@@ -267,6 +270,20 @@ class Reactor:
         _check(self.lib.prc_react_flat(self.h, _ptr(rY), _ptr(rYsrc), _ptr(T), _ptr(rE), _ptr(rEsrc),
                                        _ptr(FC), _ptr(mask), nc, ctypes.byref(dtc), ctypes.byref(tm),
                                        _stream_handle(stream)), "prc_react_flat")
+        return tm.value
+
+    def react_flat_noalloc(self, rY, T, rE, dt: float, rYsrc=None, rEsrc=None, FC=None, mask=None,
+                           time: float = 0.0, stream=None):
+        """react_flat through prc_react_flat_noalloc: the batched form with its workspace in the
+        library's static device array (no device allocation during the call)."""
+        nc = T.shape[0]
+        assert rY.shape == (self.nsp, nc) and rY.is_contiguous()
+        dtc = ctypes.c_double(dt)
+        tm = ctypes.c_double(time)
+        _check(self.lib.prc_react_flat_noalloc(self.h, _ptr(rY), _ptr(rYsrc), _ptr(T), _ptr(rE),
+                                               _ptr(rEsrc), _ptr(FC), _ptr(mask), nc,
+                                               ctypes.byref(dtc), ctypes.byref(tm),
+                                               _stream_handle(stream)), "prc_react_flat_noalloc")
         return tm.value
 
     def react_box(self, lo, hi, rY, rYsrc, T, rE, rEsrc, FC, mask, dt: float, time: float = 0.0,
