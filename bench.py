@@ -35,10 +35,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_newton launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    of this workload (profiles/k_newton_<workload>_traffic.txt), or None if not measured."""
-    path = os.path.join(ROOT, "profiles", f"k_newton_{workload}_traffic.txt")
+def measured_traffic(workload, fp32):
+    """HBM bytes per Newton-branch launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload and form (profiles/k_newton_<fp32lu|fp64lu>_<workload>_traffic.txt),
+    or None if not measured."""
+    form = "fp32lu" if fp32 else "fp64lu"
+    path = os.path.join(ROOT, "profiles", f"k_newton_{form}_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -262,7 +264,8 @@ def run_amr(args, torch, inputs, Reactor):
     act = np.flatnonzero(b["mask"])
     shape = None
     if not args.no_tune:
-        samp = act[:: max(1, act.size // 262144)][:262144]
+        k = args.tune_cells if args.tune_cells > 0 else 262144
+        samp = act[:: max(1, act.size // k)][:k]
         tuner = Reactor(wl.mech)
         tuner.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=True)
         si = torch.tensor(samp, device=dev)
@@ -354,6 +357,8 @@ def main():
                          "LiDryer and drm19, 8192 for the 53-species mechanisms)")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="timed calls of the plain form (0: --steps)")
+    ap.add_argument("--plain-warmup", type=int, default=-1,
+                    help="warm-up calls of the plain form (default: --warmup, at most --plain-steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "plain"], default=None,
                     help="time one form only (profiling runs)")
@@ -418,11 +423,16 @@ def main():
                                         args.warmup, shape=sh), shape=sh, fp32=fp32)
     if args.only != "batched":
         ps = args.plain_steps if args.plain_steps > 0 else args.steps
-        plain = time_form(torch, Reactor, wl, d0, "plain", False, ps, min(args.warmup, ps))
+        pw = args.plain_warmup if args.plain_warmup >= 0 else min(args.warmup, ps)
+        plain = time_form(torch, Reactor, wl, d0, "plain", False, ps, pw)
     if args.only:
-        o = plain if args.only == "plain" else min(forms.values(), key=lambda f: np.median(f["times"]))
-        print(json.dumps({"form": args.only, "ms_per_step": float(np.median(o["times"])) * 1e3,
-                          "cells_per_s": n / float(np.median(o["times"]))}), flush=True)
+        res = {k: {"ms_per_step": float(np.median(f["times"])) * 1e3,
+                   "cells_per_s": n / float(np.median(f["times"])),
+                   "newton_iterations_per_cell": float(f["diag"][5].mean()),
+                   "steps_per_cell": float(f["diag"][1].mean()), "failed_cells": int(f["nfail"]),
+                   "launch_shape": f.get("shape")}
+               for k, f in ([("plain_fp64", plain)] if args.only == "plain" else forms.items())}
+        print(json.dumps({"workload": wl.name, "only": args.only, "forms": res}), flush=True)
         return
 
     # oracle on every cell of the workload (or every k-th cell with --sample): the parity
@@ -482,7 +492,7 @@ def main():
     t_newton = kt["kernel_ms"]["k_newton"] * 1e-3
     launches = max(1, kt["kernel_launches"])
     achieved = W_newton / t_newton
-    traffic = measured_traffic(wl.name)
+    traffic = measured_traffic(wl.name, head["fp32"])
     dgp = plain["diag"].astype(np.float64)
     Wp = flops.work(wl.mech, dgp[2].sum(), dgp[3].sum(), dgp[4].sum(), dgp[5].sum(), dgp[1].sum())
 

@@ -136,9 +136,12 @@ def test_adjoint_vs_forward_sensitivities(adjoint_case, oracle_lib):
 
 def test_adjoint_vs_central_differences(adjoint_case):
     """Central differences (h = 1e-6) of the replayed map, summed over the 32 cells, on 64 sampled
-    reactions: within 1e-5 relative plus the differences' own round-off, 1e-8 of the summed
-    |loss| (an FP64 difference quotient carries about eps |L| / h = 2e-10 |L|); and on rho*e0
-    of cell 0."""
+    reactions: the adjoint's log-A gradient within 1e-5 of the differences' largest component,
+    component by component. The loss here is O(10) while its log-A derivatives are O(1e-6), so a
+    difference quotient at h = 1e-6 resolves the larger derivatives to about 1e-5 and the small
+    ones not at all (its round-off is ~1e-11 absolute); the tangent-linear replay in
+    test_adjoint_vs_forward_sensitivities checks every component to 1e-5 relative. And rho*e0 of
+    cell 0 to 1e-5 relative."""
     W, d, m, ct_rY, ct_T, res, om = adjoint_case
     n = d["T"].shape[0]
     R = m.nrxn
@@ -158,13 +161,13 @@ def test_adjoint_vs_central_differences(adjoint_case):
     with ThreadPoolExecutor(16) as ex:
         fds = np.array(list(ex.map(fd, js)))
     Labs = sum(abs(_loss(om, res, d, c, ct_rY, ct_T, W.dt)) for c in range(n))
-    floor = 1e-8 * Labs
     err = np.abs(g[js] - fds)
-    big = np.abs(fds) > 1e3 * floor
-    note(f"  FD logA ({js.size} reactions x {n} cells, sum |L| {Labs:.3e}): max rel err "
-         f"{(err[big] / np.abs(fds[big])).max():.2e} on the {int(big.sum())} above 1e3 x the FD "
-         f"round-off floor, max err {err.max():.2e} against the floor {floor:.2e}")
-    assert (err <= REL * np.abs(fds) + floor).all(), list(zip(js, g[js], fds))
+    scale = np.abs(fds).max()
+    big = np.abs(fds) >= 0.1 * scale
+    note(f"  FD logA ({js.size} reactions x {n} cells, sum |L| {Labs:.2e}, largest derivative "
+         f"{scale:.2e}): max err / largest {err.max() / scale:.2e}; relative error "
+         f"{(err[big] / np.abs(fds[big])).max():.2e} on the {int(big.sum())} within 10x of it")
+    assert err.max() <= REL * scale, list(zip(js, g[js], fds))
     he = 1e-6 * d["rE"][0]
     fe = (_loss(om, res, d, 0, ct_rY, ct_T, W.dt, rE0=d["rE"][0] + he)
           - _loss(om, res, d, 0, ct_rY, ct_T, W.dt, rE0=d["rE"][0] - he)) / (2 * he)
