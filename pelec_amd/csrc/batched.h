@@ -42,7 +42,13 @@ namespace pelec {
 // cells tagged by iteration i - 1 only. PH_LUFAIL0 + r: the matrix factorised during such an
 // iteration was singular; the Newton kernel of iteration i + 1 runs the convergence-failure
 // handling for it (which needs every integrator scalar, so the factorisation kernel does not).
-enum Phase : int { PH_NEWTON = 0, PH_SETUP = 1, PH_DONE = 2, PH_READY0 = 3, PH_LUFAIL0 = 6 };
+enum Phase : int {
+  PH_NEWTON = 0,
+  PH_SETUP = 1,
+  PH_DONE = 2,
+  PH_READY0 = 3,   // + list: factorised in the set-up branch of that list's iteration
+  PH_LUFAIL0 = 6   // + list: that factorisation found the Newton matrix singular
+};
 
 // Set-up lists rotate over three buffers: iteration i consumes list i%3, appends to list (i+1)%3
 // and clears list (i+2)%3 (consumed during i-1, first appended to during i+1).

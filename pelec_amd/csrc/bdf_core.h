@@ -113,6 +113,18 @@ __device__ inline void component_fence() {
   if constexpr (V::kMemory) PELEC_MEMORY_FENCE();
 }
 
+// Components per chunk of a Nordsieck pass. With the batched form's structure-of-arrays state
+// (one cell per lane, kMemory) a pass loads a chunk's history together, updates it in registers,
+// stores it and fences. One component at a time would cost a memory round trip per component and
+// pass: a component's loads cannot move above the previous component's stores, because the
+// compiler cannot prove that rows C slots apart do not alias. Private-array and group accessors
+// keep one component per iteration.
+constexpr int BDF_KCHUNK = 6;
+template <class V>
+__device__ constexpr int chunk_of() {
+  return (V::kMemory && V::kStep == 1) ? BDF_KCHUNK : 1;
+}
+
 // Column j of the Nordsieck array for component i, loaded only where j <= q.
 template <class V>
 __device__ inline void load_hist(const V& v, int i, int q, double (&z)[BDF_QMAX + 1]) {
@@ -175,8 +187,22 @@ __device__ inline double wrms_reg(const V& v, const X& x) {
 
 template <int n, class V>
 __device__ inline void set_ewt(const V& v, const BdfParams& prm) {
+  constexpr int K = chunk_of<V>();
 #pragma unroll
-  for (int i = v.c0(); i < n; i += V::kStep) v.E(i) = 1.0 / (prm.rtol * fabs(v.Z(0, i)) + prm.atolv[i]);
+  for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+    double z0[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      z0[c] = i < n ? v.Z(0, i) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      if (i < n) v.E(i) = 1.0 / (prm.rtol * fabs(z0[c]) + prm.atolv[i]);
+    }
+    component_fence<V>();
+  }
 }
 
 template <int n, class V>
@@ -187,15 +213,36 @@ __device__ inline void rescale(BdfScalars& S, const V& v) {
   for (int j = 1; j <= BDF_QMAX; ++j) f[j] = f[j - 1] * S.eta;
   const int q = S.q;
   v.rec_rescale(q, S.eta);
+  constexpr int K = chunk_of<V>();
 #pragma unroll
-  for (int i = v.c0(); i < n; i += V::kStep) {
+  for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+    double z[K][BDF_QMAX + 1];
 #pragma unroll
-    for (int j = 1; j <= BDF_QMAX; ++j)
-      if (j <= q) v.Z(j, i) *= f[j];
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+#pragma unroll
+      for (int j = 1; j <= BDF_QMAX; ++j) z[c][j] = (i < n && j <= q) ? v.Z(j, i) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+#pragma unroll
+      for (int j = 1; j <= BDF_QMAX; ++j)
+        if (i < n && j <= q) v.Z(j, i) = z[c][j] * f[j];
+    }
     component_fence<V>();
   }
   S.h = S.hscale * S.eta;
   S.hscale = S.h;
+}
+
+// Pascal-triangle update of one component's history in registers (rows 0..q).
+__device__ inline void pascal_apply(double (&z)[BDF_QMAX + 1], int q, double sign) {
+#pragma unroll
+  for (int k = 1; k <= BDF_QMAX; ++k)
+#pragma unroll
+    for (int j = BDF_QMAX; j >= k; --j)
+      z[j - 1] = (k <= q && j <= q) ? z[j - 1] + sign * z[j] : z[j - 1];
 }
 
 // Pascal-triangle update of the Nordsieck array (sign +1: predict, -1: undo the prediction),
@@ -203,16 +250,27 @@ __device__ inline void rescale(BdfScalars& S, const V& v) {
 template <int n, class V>
 __device__ inline void pascal(const V& v, int q, double sign) {
   v.rec_pascal(q, sign);
+  constexpr int K = chunk_of<V>();
 #pragma unroll
-  for (int i = v.c0(); i < n; i += V::kStep) {
-    double z[BDF_QMAX + 1];
-    load_hist(v, i, q, z);
+  for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+    double z[K][BDF_QMAX + 1];
 #pragma unroll
-    for (int k = 1; k <= BDF_QMAX; ++k)
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      if (i < n) {
+        load_hist(v, i, q, z[c]);
+      } else {
 #pragma unroll
-      for (int j = BDF_QMAX; j >= k; --j)
-        z[j - 1] = (k <= q && j <= q) ? z[j - 1] + sign * z[j] : z[j - 1];
-    store_hist(v, i, q - 1, z);
+        for (int j = 0; j <= BDF_QMAX; ++j) z[c][j] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) pascal_apply(z[c], q, sign);
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      if (i < n) store_hist(v, i, q - 1, z[c]);
+    }
     component_fence<V>();
   }
 }
@@ -282,10 +340,13 @@ __device__ inline void set_bdf(BdfScalars& S) {
   S.tq[4] = cv::NLSCOEF / S.tq[2];
 }
 
-template <int n, class V>
-__device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax) {
-  if (S.q == 2 && dq != 1) return;
-  (void)qmax;
+// Scalar part of an order change (cvIncreaseBDF / cvDecreaseBDF): the coefficients in S.l, and
+// A1 for an increase. Returns 1 (increase) or -1 (decrease) when the Nordsieck array needs the
+// component update (order_apply), 0 when it does not (a decrease from order 2). The operation is
+// recorded here.
+template <class V>
+__device__ inline int order_coeffs(BdfScalars& S, const V& v, int dq, double& A1) {
+  if (S.q == 2 && dq != 1) return 0;
   if (dq == 1) {
 #pragma unroll
     for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
@@ -306,41 +367,78 @@ __device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax)
         xiold = on ? xi : xiold;
       }
     }
-    const double A1 = (-alpha0 - alpha1) / prod;
-    const int q = S.q, L = S.L, ia = S.indx_acor;
-    v.rec_order_up(q, L, ia, A1, S.l);
+    A1 = (-alpha0 - alpha1) / prod;
+    v.rec_order_up(S.q, S.L, S.indx_acor, A1, S.l);
+    return 1;
+  }
 #pragma unroll
-    for (int i = v.c0(); i < n; i += V::kStep) {
-      const double zl = A1 * v.Z(ia, i);
-      v.Z(L, i) = zl;
+  for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
+  S.l[2] = 1.0;
+  double hsum = 0.0;
+#pragma unroll
+  for (int j = 1; j <= BDF_QMAX - 2; ++j) {
+    const bool on = j <= S.q - 2;
+    hsum = on ? hsum + S.tau[j] : hsum;
+    const double xi = hsum / S.hscale;
+#pragma unroll
+    for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xi + S.l[i - 1] : S.l[i];
+  }
+  v.rec_order_down(S.q, S.l);
+  return -1;
+}
+
+// Component part of an order change on one component's history z (rows 0..QMAX), with the
+// coefficients l of order_coeffs and the order q and L (= q + 1) before the change. Increase
+// (kind 1): z_L = A1 z_ia (za: z_ia as loaded), then z_j += l_j z_L for 2 <= j <= q.
+// Decrease (kind -1): z_j -= l_j z_q for 2 <= j < q.
+__device__ inline void order_apply(int kind, const double* l, double A1, int q, int L, double za,
+                                   double (&z)[BDF_QMAX + 1]) {
+  if (kind == 1) {
+    const double zl = A1 * za;
+#pragma unroll
+    for (int j = 2; j <= BDF_QMAX; ++j)
+      if (j <= q) z[j] = z[j] + l[j] * zl;
+#pragma unroll
+    for (int j = 1; j <= BDF_QMAX; ++j)
+      if (j == L) z[j] = zl;
+  } else if (kind == -1) {
+    const double zq = sel(z, q);
+#pragma unroll
+    for (int j = 2; j < BDF_QMAX; ++j)
+      if (j < q) z[j] = z[j] - l[j] * zq;
+  }
+}
+
+template <int n, class V>
+__device__ inline void adjust_order(BdfScalars& S, const V& v, int dq, int qmax) {
+  (void)qmax;
+  double A1 = 0.0;
+  const int kind = order_coeffs(S, v, dq, A1);
+  if (!kind) return;
+  const int q = S.q, L = S.L, ia = S.indx_acor;
+  // rows read: 2..q and ia (increase), 2..q (decrease); written: 2..q and L / 2..q-1
+  const int qr = kind == 1 ? (L > q ? L : q) : q;
+  constexpr int K = chunk_of<V>();
+#pragma unroll
+  for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+    double za[K], z[K][BDF_QMAX + 1];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      za[c] = (i < n && kind == 1) ? v.Z(ia, i) : 0.0;
+      z[c][0] = z[c][1] = 0.0;
+#pragma unroll
+      for (int j = 2; j <= BDF_QMAX; ++j) z[c][j] = (i < n && j <= q) ? v.Z(j, i) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      order_apply(kind, S.l, A1, q, L, za[c], z[c]);
 #pragma unroll
       for (int j = 2; j <= BDF_QMAX; ++j)
-        if (j <= q) v.Z(j, i) += S.l[j] * zl;
-      component_fence<V>();
+        if (i < n && j <= qr) v.Z(j, i) = z[c][j];
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < BDF_QMAX + 2; ++i) S.l[i] = 0.0;
-    S.l[2] = 1.0;
-    double hsum = 0.0;
-#pragma unroll
-    for (int j = 1; j <= BDF_QMAX - 2; ++j) {
-      const bool on = j <= S.q - 2;
-      hsum = on ? hsum + S.tau[j] : hsum;
-      const double xi = hsum / S.hscale;
-#pragma unroll
-      for (int i = j + 2; i >= 2; --i) S.l[i] = on ? S.l[i] * xi + S.l[i - 1] : S.l[i];
-    }
-    const int q = S.q;
-    v.rec_order_down(q, S.l);
-#pragma unroll
-    for (int i = v.c0(); i < n; i += V::kStep) {
-      const double zq = v.Z(q, i);
-#pragma unroll
-      for (int j = 2; j < BDF_QMAX; ++j)
-        if (j < q) v.Z(j, i) -= S.l[j] * zq;
-      component_fence<V>();
-    }
+    component_fence<V>();
   }
 }
 
@@ -363,9 +461,8 @@ __device__ inline void step_prologue(BdfScalars& S, const V& v, int qmax) {
 
 // Predict, compute BDF coefficients and gamma, decide whether the Newton matrix must be
 // (re)built (cvNls). Returns true when a set-up is needed; S.convfail is its argument.
-template <int n, class V>
-__device__ inline bool begin_attempt(BdfScalars& S, const V& v, double tstop) {
-  predict<n>(S, v, tstop);
+// begin_attempt after the prediction: coefficients, gamma and the set-up decision.
+__device__ inline bool attempt_setup(BdfScalars& S) {
   set_bdf(S);
   S.gamma = S.h / S.l[1];
   if (S.nst == 0) S.gammap = S.gamma;
@@ -374,6 +471,84 @@ __device__ inline bool begin_attempt(BdfScalars& S, const V& v, double tstop) {
   S.m = 0;
   return (S.nflag == 1) || (S.nflag == 2) || (S.nst == 0) || (S.nst >= S.nstlp + cv::MSBP) ||
          (fabs(S.gamrat - 1.0) > cv::DGMAX);
+}
+
+template <int n, class V>
+__device__ inline bool begin_attempt(BdfScalars& S, const V& v, double tstop) {
+  predict<n>(S, v, tstop);
+  return attempt_setup(S);
+}
+
+// set_ewt, step_prologue and begin_attempt after a successful step, in one pass over the
+// components: a component's error weight from z_0, then its order change, rescaling and
+// prediction -- the operations of the three calls, in their order, on one component's history
+// in registers. Returns begin_attempt's set-up decision.
+template <int n, class V>
+__device__ inline bool next_attempt(BdfScalars& S, const V& v, const BdfParams& prm,
+                                    double tstop) {
+  // step_prologue
+  S.saved_t = S.tn;
+  S.ncf = S.nef = 0;
+  S.nflag = 0;
+  const int q0 = S.q, L0 = S.L, ia = S.indx_acor;
+  int kind = 0;  // order change on the components (order_coeffs)
+  double A1 = 0.0;
+  const bool resc = S.nst > 0 && S.hprime != S.h;
+  if (resc) {
+    if (S.qprime != S.q) {
+      kind = order_coeffs(S, v, S.qprime - S.q, A1);
+      S.q = S.qprime;
+      S.L = S.q + 1;
+      S.qwait = S.L;
+    }
+  }
+  const int q = S.q;
+  double f[BDF_QMAX + 1];
+  f[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= BDF_QMAX; ++j) f[j] = f[j - 1] * S.eta;
+  if (resc) {
+    v.rec_rescale(q, S.eta);
+    S.h = S.hscale * S.eta;
+    S.hscale = S.h;
+  }
+  // begin_attempt: predict
+  S.tn += S.h;
+  if ((S.tn - tstop) * S.h > 0.0) S.tn = tstop;
+  v.rec_pascal(q, 1.0);
+  const int ql = q0 > q ? q0 : q;  // rows read (an order decrease reads row q0)
+  constexpr int K = chunk_of<V>();
+#pragma unroll
+  for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+    double za[K], z[K][BDF_QMAX + 1];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      za[c] = (i < n && kind == 1) ? v.Z(ia, i) : 0.0;
+#pragma unroll
+      for (int j = 0; j <= BDF_QMAX; ++j) z[c][j] = (i < n && j <= ql) ? v.Z(j, i) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const int i = i0 + c * V::kStep;
+      const double ew = i < n ? 1.0 / (prm.rtol * fabs(z[c][0]) + prm.atolv[i]) : 0.0;
+      order_apply(kind, S.l, A1, q0, L0, za[c], z[c]);
+      if (resc) {
+#pragma unroll
+        for (int j = 1; j <= BDF_QMAX; ++j)
+          if (j <= q) z[c][j] = z[c][j] * f[j];
+      }
+      pascal_apply(z[c], q, 1.0);
+      if (i < n) {
+        v.E(i) = ew;
+#pragma unroll
+        for (int j = 0; j <= BDF_QMAX; ++j)
+          if (j <= q) v.Z(j, i) = z[c][j];
+      }
+    }
+    component_fence<V>();
+  }
+  return attempt_setup(S);
 }
 
 // Jacobian re-evaluation decision of the dense linear solver set-up (cvDlsSetup).
@@ -449,7 +624,10 @@ __device__ inline void set_eta(BdfScalars& S) {
   }
 }
 
-// cvCompleteStep + cvPrepareNextStep + etamax update after a successful step.
+// cvCompleteStep + cvPrepareNextStep + etamax update after a successful step. The Nordsieck
+// update and the two norms of the order-change test (taken when qwait reaches 0: z_q after the
+// update, and acor against the saved correction in column qmax) are one pass over the components;
+// each norm is accumulated in component order, as a separate loop would.
 template <int n, class V>
 __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qmax) {
   S.nst++;
@@ -460,15 +638,53 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
   S.qwait--;
   const bool save_acor = S.qwait == 1 && S.q != qmax;
   v.rec_correct(S, save_acor, qmax);
+  const int q = S.q;
+  // cvPrepareNextStep's order-change test, when it runs (saved_tq5 is not touched in between:
+  // save_acor needs qwait == 1, the test qwait == 0)
+  const bool test = S.etamax != 1.0 && S.qwait == 0;
+  const bool down = test && q > 1;
+  const bool up = test && q != qmax && S.saved_tq5 != 0.0;
+  const double cquot = up ? (S.tq[5] / S.saved_tq5) * pow(S.h / S.tau[2], (double)S.L) : 0.0;
+  double sdn = 0.0, sup = 0.0;
   {
-    const int q = S.q;
+    constexpr int K = chunk_of<V>();
 #pragma unroll
-    for (int i = v.c0(); i < n; i += V::kStep) {
-      const double ai = v.A(i);
+    for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+      double a[K], e[K], zm[K], z[K][BDF_QMAX + 1];
 #pragma unroll
-      for (int j = 0; j <= BDF_QMAX; ++j)
-        if (j <= q) v.Z(j, i) += S.l[j] * ai;
-      if (save_acor) v.Z(qmax, i) = ai;
+      for (int c = 0; c < K; ++c) {
+        const int i = i0 + c * V::kStep;
+        a[c] = i < n ? v.A(i) : 0.0;
+        e[c] = (i < n && test) ? v.E(i) : 0.0;
+        zm[c] = (i < n && up) ? v.Z(qmax, i) : 0.0;
+        if (i < n) {
+          load_hist(v, i, q, z[c]);
+        } else {
+#pragma unroll
+          for (int j = 0; j <= BDF_QMAX; ++j) z[c][j] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const int i = i0 + c * V::kStep;
+        if (i < n) {
+#pragma unroll
+          for (int j = 0; j <= BDF_QMAX; ++j)
+            if (j <= q) z[c][j] = z[c][j] + S.l[j] * a[c];
+          if (down) {
+            const double w = sel(z[c], q) * e[c];
+            sdn += w * w;
+          }
+          if (up) {
+            const double w = (a[c] - cquot * zm[c]) * e[c];
+            sup += w * w;
+          }
+#pragma unroll
+          for (int j = 0; j <= BDF_QMAX; ++j)
+            if (j <= q) v.Z(j, i) = z[c][j];
+          if (save_acor) v.Z(qmax, i) = a[c];
+        }
+      }
       component_fence<V>();
     }
   }
@@ -491,26 +707,13 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
     } else {
       S.qwait = 2;
       double etaqm1 = 0.0;
-      if (S.q > 1) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = v.c0(); i < n; i += V::kStep) {
-          const double w = v.Z(S.q, i) * v.E(i);
-          s += w * w;
-        }
-        const double ddn = sqrt(v.sum(s) / n) * S.tq[1];
+      if (down) {
+        const double ddn = sqrt(v.sum(sdn) / n) * S.tq[1];
         etaqm1 = 1.0 / (pow(cv::BIAS1 * ddn, 1.0 / S.q) + cv::ADDON);
       }
       double etaqp1 = 0.0;
-      if (S.q != qmax && S.saved_tq5 != 0.0) {
-        const double cquot = (S.tq[5] / S.saved_tq5) * pow(S.h / S.tau[2], (double)S.L);
-        double s = 0.0;
-#pragma unroll
-        for (int i = v.c0(); i < n; i += V::kStep) {
-          const double w = (v.A(i) - cquot * v.Z(qmax, i)) * v.E(i);
-          s += w * w;
-        }
-        const double dup = sqrt(v.sum(s) / n) * S.tq[3];
+      if (up) {
+        const double dup = sqrt(v.sum(sup) / n) * S.tq[3];
         etaqp1 = 1.0 / (pow(cv::BIAS3 * dup, 1.0 / (S.L + 1)) + cv::ADDON);
       }
       const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
@@ -527,8 +730,22 @@ __device__ inline void finish_step(BdfScalars& S, const V& v, double dsm, int qm
         S.eta = etaqp1;
         S.qprime = S.q + 1;
         v.rec_mark_save();  // the correction just applied is also saved in column qmax
+        constexpr int K = chunk_of<V>();
 #pragma unroll
-        for (int i = v.c0(); i < n; i += V::kStep) v.Z(qmax, i) = v.A(i);
+        for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+          double a[K];
+#pragma unroll
+          for (int c = 0; c < K; ++c) {
+            const int i = i0 + c * V::kStep;
+            a[c] = i < n ? v.A(i) : 0.0;
+          }
+#pragma unroll
+          for (int c = 0; c < K; ++c) {
+            const int i = i0 + c * V::kStep;
+            if (i < n) v.Z(qmax, i) = a[c];
+          }
+          component_fence<V>();
+        }
       }
       set_eta(S);
     }
@@ -604,13 +821,41 @@ __device__ inline int newton_correct(BdfScalars& S, const V& v, const FT& ftemp,
     for (int i = v.c0(); i < n; i += V::kStep) tempv[i] *= sc;
   }
   S.nni++;
-  const double del = wrms_reg<n>(v, tempv);
+  // one pass: the WRMS norm of the correction (del), acor += correction, and the WRMS norm of the
+  // updated acor (cvNlsNewton's acnrm), each sum in component order
+  double sdel = 0.0, sacor = 0.0;
+  {
+    constexpr int K = chunk_of<V>();
 #pragma unroll
-  for (int i = v.c0(); i < n; i += V::kStep) v.A(i) += tempv[i];
+    for (int i0 = v.c0(); i0 < n; i0 += V::kStep * K) {
+      double a[K], e[K];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const int i = i0 + c * V::kStep;
+        a[c] = i < n ? v.A(i) : 0.0;
+        e[c] = i < n ? v.E(i) : 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const int i = i0 + c * V::kStep;
+        if (i < n) {
+          const double w = tempv[i] * e[c];
+          sdel += w * w;
+          const double an = a[c] + tempv[i];
+          v.A(i) = an;
+          const double wa = an * e[c];
+          sacor += wa * wa;
+        }
+      }
+      component_fence<V>();
+    }
+  }
+  const double del = sqrt(v.sum(sdel) / n);
+  const double acn = sqrt(v.sum(sacor) / n);
   if (S.m > 0) S.crate = fmax(cv::CRDOWN * S.crate, del / S.delp);
   const double dcon = del * fmin(1.0, S.crate) / S.tq[4];
   if (dcon <= 1.0) {
-    S.acnrm = (S.m == 0) ? del : wrms_acor<n>(v);
+    S.acnrm = (S.m == 0) ? del : acn;
     S.jcur = 0;
     return 0;
   }

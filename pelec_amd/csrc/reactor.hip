@@ -415,6 +415,9 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
     const int e = occupancy_setup<LUT>(r);
     if (e) return e;
   }
+  // every workspace array the kernels of this shape index must be in the argument block
+  if (!a.J || !a.LU || !a.perm || !a.ring || !a.remap || (r->lu_fp32 && !a.dexp))
+    return PRC_ESTATE;
   HIP_OK(hipMemcpyAsync(dargs, &a, sizeof(Args), hipMemcpyHostToDevice, s));
   // the first fill: cells 0 .. count0-1 into slots 0 .. count0-1, each slot its own matrix block
   const int64_t C = a.C;
@@ -774,6 +777,7 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "prc.coop_tail_slots")) {
     if (v < 0 || (v && !COOP)) return PRC_EINVAL;
     r->shape.tail_slots = v;
+
   } else if (!std::strcmp(key, "prc.blocks_tail")) {
     if (v < 0 || v > 64) return PRC_EINVAL;
     r->shape.bpc_tail = v;
@@ -818,6 +822,7 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
     *v = r->shape.coop;
   else if (!std::strcmp(key, "prc.coop_tail_slots"))
     *v = r->shape.tail_slots;
+
   else if (!std::strcmp(key, "prc.blocks_tail"))
     *v = r->shape.bpc_tail > 0 ? r->shape.bpc_tail : r->occ_tail;
   else if (!std::strcmp(key, "prc.form"))

@@ -457,6 +457,44 @@ __global__ void __launch_bounds__(64) k_lu_rows(const Args* __restrict__ ap, int
   }
 }
 
+// b <- M^{-1} b with the factors of matrix block mb (one lane's cell): FP64 factors, or FP32
+// factors of the equilibrated D M D^-1 followed by NREF FP64 refinement steps against the FP64
+// Jacobian (residual b - (x - gammap J x)). gbl: the lane's LDS column (stride 64).
+template <typename LUT, int NREF>
+__device__ inline void newton_solve(const Args& a, int64_t mb, double gammap, double* b,
+                                    double* gbl) {
+  const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV);
+  const int32_t* pm = a.perm + mb * NV;
+  if constexpr (sizeof(LUT) == sizeof(double)) {
+    lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
+  } else if constexpr (NREF == 0) {
+    lu_solve_perm<NV, LUT>(lu, pm, b, gbl, a.dexp + mb * NV);
+  } else {
+    const double* Jg = a.J + mb * (int64_t)(NV * NV);
+    const int32_t* dexp = a.dexp + mb * NV;
+    double x[NV], res[NV];
+#pragma unroll
+    for (int e = 0; e < NV; ++e) x[e] = b[e];
+    lu_solve_perm<NV, LUT>(lu, pm, x, gbl, dexp);
+#pragma unroll
+    for (int it = 0; it < NREF; ++it) {
+#pragma unroll
+      for (int e = 0; e < NV; ++e) {
+        double jx = 0.0;
+#pragma unroll
+        for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
+        res[e] = b[e] - (x[e] - gammap * jx);
+        PELEC_MEMORY_FENCE();
+      }
+      lu_solve_perm<NV, LUT>(lu, pm, res, gbl, dexp);
+#pragma unroll
+      for (int e = 0; e < NV; ++e) x[e] += res[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NV; ++e) b[e] = x[e];
+  }
+}
+
 // One Newton correction per Newton-phase slot; on convergence the error test, step completion,
 // order/step choice and the next attempt's prediction.
 // NREF: FP64 refinement steps after each FP32-LU solve (FP64 factors: none)
@@ -512,40 +550,9 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
 #pragma unroll
           for (int e = 0; e < NV; ++e) v.A(e) = 0.0;
         }
-        const LUT* lu = reinterpret_cast<const LUT*>(a.LU) + mb * (int64_t)(NV * NV);
-        const int32_t* pm = a.perm + mb * NV;
-        const double* Jg = a.J + mb * (int64_t)(NV * NV);
-        const int32_t* dexp = a.dexp + mb * NV;
         const double gammap = S.gammap;
         double* gbl = gb + lane;
-        auto solve = [&](double* b) {
-          if constexpr (sizeof(LUT) == sizeof(double)) {
-            lu_solve_perm<NV, LUT>(lu, pm, b, gbl);
-          } else if constexpr (NREF == 0) {
-            lu_solve_perm<NV, LUT>(lu, pm, b, gbl, dexp);
-          } else {
-            double x[NV], res[NV];
-#pragma unroll
-            for (int e = 0; e < NV; ++e) x[e] = b[e];
-            lu_solve_perm<NV, LUT>(lu, pm, x, gbl, dexp);
-#pragma unroll
-            for (int it = 0; it < NREF; ++it) {
-#pragma unroll
-              for (int e = 0; e < NV; ++e) {
-                double jx = 0.0;
-#pragma unroll
-                for (int f2 = 0; f2 < NV; ++f2) jx += Jg[e * NV + f2] * x[f2];
-                res[e] = b[e] - (x[e] - gammap * jx);
-                PELEC_MEMORY_FENCE();
-              }
-              lu_solve_perm<NV, LUT>(lu, pm, res, gbl, dexp);
-#pragma unroll
-              for (int e = 0; e < NV; ++e) x[e] += res[e];
-            }
-#pragma unroll
-            for (int e = 0; e < NV; ++e) b[e] = x[e];
-          }
-        };
+        auto solve = [&](double* b) { newton_solve<LUT, NREF>(a, mb, gammap, b, gbl); };
         r = newton_correct<NV>(S, v, ftemp, solve);
       }
       int status = -1;  // <0 running, else final status
@@ -570,9 +577,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
             } else if (S.nst >= a.prm.max_steps) {
               status = CELL_MAXSTEPS;
             } else {
-              set_ewt<NV>(v, a.prm);
-              step_prologue<NV>(S, v, a.prm.qmax);
-              attempt = true;
+              need_setup = next_attempt<NV>(S, v, a.prm, a.dt);
             }
           } else {
             const int e = error_test_fail<NV>(S, v, dsm, a.prm.qmax);
@@ -807,9 +812,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
             } else if (S.nst >= a.prm.max_steps) {
               status = CELL_MAXSTEPS;
             } else {
-              set_ewt<NV>(v, a.prm);
-              step_prologue<NV>(S, v, a.prm.qmax);
-              attempt = true;
+              need_setup = next_attempt<NV>(S, v, a.prm, a.dt);
             }
           } else {
             const int e = error_test_fail<NV>(S, v, dsm, a.prm.qmax);
