@@ -64,7 +64,8 @@ enum Ctr : int {
   CT_CUR = 7,      // state buffer holding the live slots (0/1)
   CT_NEWACTIVE = 8,
   CT_FTAIL = 9,    // matrix blocks returned to the free ring (its tail; the host tracks the head)
-  CT_COUNT = 10
+  CT_JAC0 = 10,    // lengths of Jacobian lists 0..2: the set-up entries that need a Jacobian
+  CT_COUNT = 13
 };
 
 __host__ __device__ constexpr int list_next(int r) { return r == NLISTS - 1 ? 0 : r + 1; }

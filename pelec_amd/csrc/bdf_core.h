@@ -558,6 +558,20 @@ __device__ inline bool jac_needed(const BdfScalars& S) {
          (S.convfail == 2 && dgamma < cv::DGMAX_LS) || (S.convfail == 1);
 }
 
+// The Jacobian decision of a requested set-up with its bookkeeping (cvDlsSetup: nje, nstlj,
+// jcur), taken when the set-up is requested; nothing it reads changes before the set-up runs.
+__device__ inline bool jac_decide(BdfScalars& S) {
+  const bool need = jac_needed(S);
+  if (need) {
+    S.nje++;
+    S.nstlj = S.nst;
+    S.jcur = 1;
+  } else {
+    S.jcur = 0;
+  }
+  return need;
+}
+
 // Bookkeeping after the Newton matrix was rebuilt.
 __device__ inline void after_setup(BdfScalars& S) {
   S.nsetups++;

@@ -232,6 +232,7 @@ size_t workspace_layout(Args& L, char* base, int64_t C, bool lu_fp32, bool has_s
   L.perm = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * NV * C));
   L.dexp = lu_fp32 ? reinterpret_cast<int32_t*>(take(sizeof(int32_t) * NV * C)) : nullptr;
   for (int b = 0; b < NLISTS; ++b) L.list[b] = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
+  for (int b = 0; b < NLISTS; ++b) L.jlist[b] = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
   L.remap = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
   L.ring = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * C));
   return off;
@@ -418,6 +419,8 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
   // every workspace array the kernels of this shape index must be in the argument block
   if (!a.J || !a.LU || !a.perm || !a.ring || !a.remap || (r->lu_fp32 && !a.dexp))
     return PRC_ESTATE;
+  for (int b = 0; b < NLISTS; ++b)
+    if (!a.list[b] || !a.jlist[b]) return PRC_ESTATE;
   HIP_OK(hipMemcpyAsync(dargs, &a, sizeof(Args), hipMemcpyHostToDevice, s));
   // the first fill: cells 0 .. count0-1 into slots 0 .. count0-1, each slot its own matrix block
   const int64_t C = a.C;
@@ -557,6 +560,7 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   a.perm = L.perm;
   a.dexp = L.dexp;
   for (int b = 0; b < NLISTS; ++b) a.list[b] = L.list[b];
+  for (int b = 0; b < NLISTS; ++b) a.jlist[b] = L.jlist[b];
   a.remap = L.remap;
   a.ring = L.ring;
   a.has_src = has_src ? 1 : 0;

@@ -70,6 +70,7 @@ struct Args {
   int32_t* perm;   // per matrix block n
   int32_t* dexp;   // per matrix block n: equilibration exponents of FP32 factors (scale_exp)
   int32_t* list[NLISTS];
+  int32_t* jlist[NLISTS];  // the entries of list[i] whose set-up evaluates a Jacobian
   int32_t* remap;
   int32_t* ring;   // C: free matrix blocks (tail counter CT_FTAIL, head kept by the host)
   int has_src;
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap, int64
         for (int e = 0; e < NV; ++e) v.Z(1, e) *= S.h;
         step_prologue<NV>(S, v, a.prm.qmax);
         begin_attempt<NV>(S, v, a.dt);  // nst == 0: set-up required
+        jac_decide(S);                  // nst == 0: with a Jacobian
         store_scalars(S, st.sc, a.C, slot);
         st.phase[slot] = PH_SETUP;
       }
@@ -343,48 +345,39 @@ __global__ void __launch_bounds__(64) k_start(const Args* __restrict__ ap, int64
   }
   wave_count(&a.ctr[CT_STARTED], started);
   wave_append(a.list[0], &a.ctr[CT_SETUP0], started, (int32_t)slot);
+  wave_append(a.jlist[0], &a.ctr[CT_JAC0], started, (int32_t)slot);
   wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, unused, mblk);
 }
 
-// Jacobians of the set-up list entries that need one (cvDlsSetup's jbad decision). A template so
-// that mechanisms whose Jacobian runs in the cooperative kernel do not build it.
+// The tn word of a slot's integrator scalars.
+__device__ inline double slot_tn(const StateBuf& st, int64_t C, int64_t s) {
+  constexpr int W_TN = offsetof(BdfScalars, tn) / 8;
+  double tn;
+  const uint64_t w = st.sc[W_TN * C + s];
+  __builtin_memcpy(&tn, &w, 8);
+  return tn;
+}
+
+// Jacobians of the set-up list entries that need one: the Jacobian list, which the kernel that
+// requested the set-up filled after cvDlsSetup's decision (jac_decide). A template so that
+// mechanisms whose Jacobian runs in the cooperative kernel do not build it.
 template <int UNUSED = 0>
 __global__ void __launch_bounds__(64) k_jac(const Args* __restrict__ ap, int par) {
   const Args& a = *ap;
-  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const int64_t cnt = (int64_t)a.ctr[CT_JAC0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   for (int64_t base = (int64_t)blockIdx.x * 64; base < cnt; base += (int64_t)gridDim.x * 64) {
     const int64_t idx = base + threadIdx.x;
     if (idx >= cnt) continue;
-    const int64_t s = a.list[par][idx];
-    // bookkeeping first, so no integrator scalar is live across the Jacobian
-    bool need;
-    double tn;
-    {
-      BdfScalars S;
-      load_scalars(S, st.sc, a.C, s);
-      need = jac_needed(S);
-      if (need) {
-        S.nje++;
-        S.nstlj = S.nst;
-        S.jcur = 1;
-      } else {
-        S.jcur = 0;
-      }
-      tn = S.tn;
-      store_scalars(S, st.sc, a.C, s);
-    }
-    PELEC_MEMORY_FENCE();
-    if (need) {
-      const int64_t mb = st.mb[s];
-      const SoAV<NV> v = soa(st, a.C, s);
-      double y[NV];
+    const int64_t s = a.jlist[par][idx];
+    const int64_t mb = st.mb[s];
+    const SoAV<NV> v = soa(st, a.C, s);
+    double y[NV];
 #pragma unroll
-      for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e);
-      const double* src = a.has_src ? st.src + s : nullptr;
-      Model::template jac<double>(tn, y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
-                                  a.J + mb * (int64_t)(NV * NV));
-    }
+    for (int e = 0; e < NV; ++e) y[e] = v.Z(0, e);
+    const double* src = a.has_src ? st.src + s : nullptr;
+    Model::template jac<double>(slot_tn(st, a.C, s), y, st.rhoe[s], st.rhoe[a.C + s], src, a.C,
+                                a.J + mb * (int64_t)(NV * NV));
   }
 }
 
@@ -506,7 +499,10 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
   __shared__ BdfScalars scl[64];
   const Args& a = *ap;
   // list (par+2)%3 was consumed by the previous iteration and is first appended to by the next
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctr[CT_SETUP0 + list_prev(par)] = 0;
+    a.ctr[CT_JAC0 + list_prev(par)] = 0;
+  }
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
   const int lane = threadIdx.x;
@@ -520,7 +516,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     // singular Newton matrix in the previous iteration's set-up: CVODE's convergence-failure
     // handling, as for a Newton failure with a current Jacobian (r = 3 below)
     const bool lufail = ph == PH_LUFAIL0 + list_prev(par);
-    bool need_setup = false, done = false;
+    bool need_setup = false, need_jac = false, done = false;
     if (act || lufail) {
       const SoAV<NV> v = soa(st, a.C, s);
       const int64_t g = st.cell[s];
@@ -607,6 +603,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
                    st.rhoe[a.C + s]);
       } else if (need_setup) {
         st.phase[s] = PH_SETUP;
+        need_jac = jac_decide(S);
       } else if (ph != PH_NEWTON) {
         st.phase[s] = PH_NEWTON;
       }
@@ -614,6 +611,7 @@ __global__ void __launch_bounds__(64) k_newton(const Args* __restrict__ ap, int 
     }
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup,
                 (int32_t)s);
+    wave_append(a.jlist[list_next(par)], &a.ctr[CT_JAC0 + list_next(par)], need_jac, (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done);
     wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, done, done ? st.mb[s] : 0);
   }
@@ -671,36 +669,19 @@ __global__ void __launch_bounds__(BS) k_jac_g(const Args* __restrict__ ap, int p
   constexpr int CPB = BS / G;
   __shared__ double lds[CPB * GM::JAC_WORDS];
   const Args& a = *ap;
-  const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
+  const int64_t cnt = (int64_t)a.ctr[CT_JAC0 + par];
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int l = threadIdx.x & (G - 1);
   double* sh = lds + (threadIdx.x / G) * GM::JAC_WORDS;
   for (int64_t base = (int64_t)blockIdx.x * CPB; base < cnt; base += (int64_t)gridDim.x * CPB) {
     const int64_t idx = base + threadIdx.x / G;
     if (idx >= cnt) continue;
-    const int64_t s = a.list[par][idx];
-    bool need;
-    double tn;
-    {
-      BdfScalars S;
-      load_scalars(S, st.sc, a.C, s);
-      need = jac_needed(S);
-      if (need) {
-        S.nje++;
-        S.nstlj = S.nst;
-        S.jcur = 1;
-      } else {
-        S.jcur = 0;
-      }
-      tn = S.tn;
-      if (l == 0) store_scalars(S, st.sc, a.C, s);
-    }
-    if (need) {
-      const int64_t mb = st.mb[s];
-      const double yl = l < NV ? st.zn[(int64_t)l * a.C + s] : 0.0;
-      const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
-      GM::jac(tn, yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh, a.J + mb * (int64_t)(NV * NV));
-    }
+    const int64_t s = a.jlist[par][idx];
+    const int64_t mb = st.mb[s];
+    const double yl = l < NV ? st.zn[(int64_t)l * a.C + s] : 0.0;
+    const double srcl = (a.has_src && l < NSP) ? st.src[(int64_t)l * a.C + s] : 0.0;
+    GM::jac(slot_tn(st, a.C, s), yl, l, st.rhoe[s], st.rhoe[a.C + s], srcl, sh,
+            a.J + mb * (int64_t)(NV * NV));
   }
 }
 
@@ -714,7 +695,10 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
   // be replicated on every lane of the group and more than halve the occupancy)
   __shared__ BdfScalars scl[CPB];
   const Args& a = *ap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr[CT_SETUP0 + list_prev(par)] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctr[CT_SETUP0 + list_prev(par)] = 0;
+    a.ctr[CT_JAC0 + list_prev(par)] = 0;
+  }
   const StateBuf& st = a.st[a.ctr[CT_CUR]];
   const int64_t nact = (int64_t)a.ctr[CT_NACTIVE];
   const int l = threadIdx.x & (G - 1);
@@ -724,7 +708,7 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
     const int ph = s < nact ? st.phase[s] : PH_DONE;
     const bool act = ph == PH_NEWTON || ph == PH_READY0 + list_prev(par);
     const bool lufail = ph == PH_LUFAIL0 + list_prev(par);  // as in k_newton
-    bool need_setup = false, done = false;
+    bool need_setup = false, need_jac = false, done = false;
     if (act || lufail) {
       const GrpV<NV, G> v{{}, st.zn, st.acor, st.ewt, a.C, s, l};
       const int64_t g = st.cell[s];
@@ -838,15 +822,20 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
           if (status != CELL_OK) atomicAdd(&a.ctr[CT_FAILED], 1ull);
         }
         store_cell_g<GM>(a, g, status, counters(S), l < NV ? v.Z(0, l) : 0.0, l, rhoe0, rhoesrc, sh);
-      } else if (l == 0) {
-        if (need_setup)
-          st.phase[s] = PH_SETUP;
-        else if (ph != PH_NEWTON)
-          st.phase[s] = PH_NEWTON;
+      } else {
+        if (need_setup) need_jac = jac_decide(S);  // every lane, on the group's one record
+        if (l == 0) {
+          if (need_setup)
+            st.phase[s] = PH_SETUP;
+          else if (ph != PH_NEWTON)
+            st.phase[s] = PH_NEWTON;
+        }
       }
       store_scalars_group<G>(S, st.sc, a.C, s, l);
     }
     wave_append(a.list[list_next(par)], &a.ctr[CT_SETUP0 + list_next(par)], need_setup && l == 0,
+                (int32_t)s);
+    wave_append(a.jlist[list_next(par)], &a.ctr[CT_JAC0 + list_next(par)], need_jac && l == 0,
                 (int32_t)s);
     wave_count(&a.ctr[CT_DONE], done && l == 0);
     wave_push_ring(a.ring, &a.ctr[CT_FTAIL], a.C, done && l == 0, done ? st.mb[s] : 0);
@@ -905,6 +894,10 @@ __global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int p
   for (int64_t idx = (int64_t)blockIdx.x * 64 + threadIdx.x; idx < cnt;
        idx += (int64_t)gridDim.x * 64)
     a.list[par][idx] = a.remap[a.list[par][idx]];
+  const int64_t cj = (int64_t)a.ctr[CT_JAC0 + par];
+  for (int64_t idx = (int64_t)blockIdx.x * 64 + threadIdx.x; idx < cj;
+       idx += (int64_t)gridDim.x * 64)
+    a.jlist[par][idx] = a.remap[a.jlist[par][idx]];
 }
 
 __global__ void k_compact_done(const Args* __restrict__ ap) {

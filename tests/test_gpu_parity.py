@@ -508,6 +508,36 @@ def oracle_fma(oracle_lib):
     return get
 
 
+def test_react_rk64_workload_tolerance_vs_oracle(oracles, oracle_fma):
+    """ReactorRK64 on drm19 at the BDF workload's own tolerances (rtol 1e-6, atol 1e-10; dt 1e-6,
+    512 cells). At this tolerance a round-off difference flips accept/reject decisions of the
+    explicit controller, and trajectories separate by its truncation error: every cell is held to
+    max(gate, 8 x) the spread the oracle shows against its own FMA build and one-ulp inputs
+    (check_with_spread), and cells whose step and RHS counts equal the oracle's to the gate."""
+    W = inputs.WORKLOADS["drm19_64"]
+    n, dt, rtol, atol = 512, 1e-6, W.rtol, W.atol
+    d = inputs.make_inputs("drm19_64", n)
+    r = Reactor(W.mech, chem_integrator="ReactorRK64")
+    r.configure(rtol=rtol, atol=atol)
+    rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+    diag = torch.zeros((8, n), dtype=torch.int32, device=DEV)
+    r.set_diagnostics(diag)
+    note(f"react_rk64 drm19 n={n} dt={dt} rtol={rtol}")
+    r.react_flat(rY, T, rE, dt)
+    torch.cuda.synchronize()
+    assert r.last_failures() == 0
+    r.close()
+    kw = dict(rtol=rtol, atol=atol, nsub_guess=10, nsub_min=5, nsub_max=100000, nthreads=8)
+    om, omf = oracles[W.mech], oracle_fma(W.mech)
+    ref = om.react_rk64(d["rY"], d["T"], d["rE"], dt, **kw)
+    alts = [omf.react_rk64(d["rY"], d["T"], d["rE"], dt, **kw)]
+    for sgn in (1.0, -1.0):
+        alts.append(om.react_rk64(d["rY"], d["T"], d["rE"] + sgn * np.spacing(d["rE"]), dt, **kw))
+    assert ref["nfail"] == 0
+    dg = diag.cpu().numpy()
+    check_with_spread("rk64 drm19 rtol 1e-6", T.cpu().numpy(), rY.cpu().numpy(), ref, alts, diag=dg)
+
+
 @pytest.mark.parametrize("knob", ["typ_vals", "max_order4"])
 @pytest.mark.parametrize("form,lu_fp32", [("plain", False), ("batched", True)])
 def test_production_knobs_vs_oracle(knob, form, lu_fp32, oracles, oracle_fma):
