@@ -108,7 +108,10 @@ int prc_set_param_real(prc_reactor* r, const char* key, double value);
 /* (launch shape of the batched form: prc.iters_per_graph (multiple of 3), prc.cooperative (bit 0:
  * the Newton kernel, bit 1: the Jacobian kernel run one cell per 32-lane group instead of one
  * cell per lane), prc.blocks_newton, prc.blocks_jac, prc.blocks_lu (blocks per CU, 0 =
- * occupancy-derived), prc.compact_pct, prc.coop_tail_slots (once at most this many slots are in
+ * occupancy-derived), prc.compact_pct (compact the work queue when fewer than this percentage
+ * of its slots are alive), prc.compact_every (the graph itself checks for a compaction after
+ * every this many iterations while no cell waits for a slot; 0 = only between replays),
+ * prc.coop_tail_slots (once at most this many slots are in
  * use, the Newton kernel runs one cell per 32-lane group; 0 = never) and prc.blocks_tail (its
  * blocks per CU), prc.max_slots (slots of the batched form's work queue: 0 = one per cell while
  * half the free device memory holds them; a call with more cells than slots refills the slots
@@ -124,7 +127,8 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* value);
  * shapes -- the kernel forms (prc.cooperative, prc.coop_tail_slots), iterations per captured
  * graph (prc.iters_per_graph), blocks per CU of the Newton,
  * Jacobian and LU kernels (prc.blocks_newton / _jac / _lu), the compaction threshold
- * (prc.compact_pct) -- by coordinate descent from the occupancy-derived defaults, keeps the
+ * and interval (prc.compact_pct, prc.compact_every) -- by coordinate descent from the
+ * occupancy-derived defaults, keeps the
  * fastest in the reactor and returns its time (ms) in *best_ms. Synchronises the stream. No
  * counterpart in the reference (its GPU launch shapes are fixed at compile time). */
 int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double* T,

@@ -65,7 +65,9 @@ enum Ctr : int {
   CT_NEWACTIVE = 8,
   CT_FTAIL = 9,    // matrix blocks returned to the free ring (its tail; the host tracks the head)
   CT_JAC0 = 10,    // lengths of Jacobian lists 0..2: the set-up entries that need a Jacobian
-  CT_COUNT = 13
+  CT_INGRAPH = 13, // 1: compactions inside the graph allowed (no cell of the call waits for a slot)
+  CT_GATE = 14,    // the graph's compaction check of the current iteration decided to compact
+  CT_COUNT = 15
 };
 
 __host__ __device__ constexpr int list_next(int r) { return r == NLISTS - 1 ? 0 : r + 1; }

@@ -54,10 +54,14 @@ struct Shape {
   // 7.3e-5 K of the oracle, as with FP64 factors); with none, 1333 of its 262144 cells took
   // other step decisions and ended up to 0.056 K off.
   int refine = 1;
+  // prc.compact_every: the graph checks for a compaction after every this many iterations (with
+  // the host's rule, compact_pct; only while no cell waits for a slot), so compactions need not
+  // wait for the end of a replay. 0: only the host compacts, between replays.
+  int compact_every = 0;
   bool operator==(const Shape& o) const {
     return iters == o.iters && coop == o.coop && bpc_newton == o.bpc_newton && bpc_jac == o.bpc_jac &&
            bpc_lu == o.bpc_lu && compact_pct == o.compact_pct && tail_slots == o.tail_slots &&
-           bpc_tail == o.bpc_tail && refine == o.refine;
+           bpc_tail == o.bpc_tail && refine == o.refine && compact_every == o.compact_every;
   }
 };
 
@@ -281,6 +285,20 @@ int newton_grid(const prc_reactor* r, bool tail) {
 
 // One graph's worth of batched iterations enqueued on s (set-up branch) and s2 (Newton branch),
 // forked and joined every iteration; the kernels read the call's arguments at `args`.
+// The graph's own compaction check after iteration `it` (prc.compact_every): the gate kernel
+// takes the host's decision on the device, the three compaction kernels follow it. The set-up
+// list pending at that point is the one the iteration appended to.
+bool compact_here(const Shape& sh, int it) {
+  return sh.compact_every > 0 && (it + 1) % sh.compact_every == 0;
+}
+void enqueue_compaction(prc_reactor* r, hipStream_t s, const Args* args, int pending_list,
+                        int grid) {
+  hipLaunchKernelGGL(k_compact_gate, dim3(1), dim3(64), 0, s, args, r->shape.compact_pct);
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(64), 0, s, args, 1);
+  hipLaunchKernelGGL(k_remap, dim3(grid), dim3(64), 0, s, args, pending_list, 1);
+  hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, args, 1);
+}
+
 template <typename LUT>
 bool enqueue_iterations(prc_reactor* r, hipStream_t s, hipStream_t s2, const Args* args,
                         bool tail) {
@@ -300,6 +318,7 @@ bool enqueue_iterations(prc_reactor* r, hipStream_t s, hipStream_t s2, const Arg
     hipLaunchKernelGGL(lu_kernel<LUT>(), dim3(gl), dim3(64), 0, s, args, par);
     ok = hipEventRecord(r->ev_join, s2) == hipSuccess &&
          hipStreamWaitEvent(s, r->ev_join, 0) == hipSuccess;
+    if (ok && compact_here(r->shape, it)) enqueue_compaction(r, s, args, list_next(par), gn);
   }
   return ok && hipGetLastError() == hipSuccess;
 }
@@ -384,6 +403,7 @@ int eager_iterations(prc_reactor* r, hipStream_t s, bool tail) {
     HIP_OK(hipEventRecord(e[4], s));
     HIP_OK(hipEventRecord(r->ev_join, s2));
     HIP_OK(hipStreamWaitEvent(s, r->ev_join, 0));
+    if (compact_here(r->shape, it)) enqueue_compaction(r, s, r->d_args, list_next(par), gn);
   }
   HIP_OK(hipGetLastError());
   return PRC_OK;
@@ -425,7 +445,9 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
   // the first fill: cells 0 .. count0-1 into slots 0 .. count0-1, each slot its own matrix block
   const int64_t C = a.C;
   const int64_t count0 = std::min(C, a.ncells);
-  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, dctr, count0);
+  // compactions inside the graph while no cell of the call waits for a slot
+  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, dctr, count0,
+                     count0 == a.ncells ? 1 : 0);
   hipLaunchKernelGGL(k_start, dim3((unsigned)((count0 + 63) / 64)), dim3(64), 0, s, dargs,
                      (int64_t)0, count0, (int64_t)0, (int64_t)-1);
   HIP_OK(hipGetLastError());
@@ -468,6 +490,7 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
     }
     const int64_t started = (int64_t)r->h_ctr[CT_STARTED];
     const int64_t done = (int64_t)r->h_ctr[CT_DONE];
+    nactive = (int64_t)r->h_ctr[CT_NACTIVE];  // the graph may have compacted
     const bool pending = next < a.ncells;
     if (done >= started && !pending) break;
     if (iters > max_iters) return PRC_ESTATE;
@@ -477,9 +500,9 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
                    "queued %lld\n", (long long)iters, (long long)started, (long long)done,
                    (long long)alive, (long long)nactive, (long long)(a.ncells - next));
     if (alive * 100 < nactive * r->shape.compact_pct && (nactive > 4096 || pending)) {
-      hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, dargs);
-      hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, dargs, 0);
-      hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, dargs);
+      hipLaunchKernelGGL(k_compact, dim3(gn), dim3(64), 0, s, dargs, 0);
+      hipLaunchKernelGGL(k_remap, dim3(gn), dim3(64), 0, s, dargs, 0, 0);
+      hipLaunchKernelGGL(k_compact_done, dim3(1), dim3(64), 0, s, dargs, 0);
       HIP_OK(hipGetLastError());
       nactive = alive;
       if (pending) {
@@ -493,6 +516,8 @@ int run_batched(prc_reactor* r, Args& a, hipStream_t s, Args* dargs, unsigned lo
         next += k;
         head += k;
         nactive = alive + k;
+        if (next == a.ncells)  // the queue is empty: the graph may compact from now on
+          hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, s, dctr, (int)CT_INGRAPH, 1ull);
       }
     }
   }
@@ -518,7 +543,7 @@ int launch(prc_reactor* r, Args& a, bool has_src, hipStream_t s, bool reset_coun
   fill_params(r, a, r->d_ctr);
   if (a.ncells == 0 || r->form == 0 || r->integrator == 1) {
     if (reset_counters) {
-      hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, (int64_t)0);
+      hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, r->d_ctr, (int64_t)0, 0);
       HIP_OK(hipGetLastError());
     }
     if (a.ncells == 0) return PRC_OK;
@@ -797,6 +822,9 @@ int prc_set_param_int(prc_reactor* r, const char* key, int v) {
   } else if (!std::strcmp(key, "prc.compact_pct")) {
     if (v < 1 || v > 100) return PRC_EINVAL;
     r->shape.compact_pct = v;
+  } else if (!std::strcmp(key, "prc.compact_every")) {
+    if (v < 0 || v > 300) return PRC_EINVAL;
+    r->shape.compact_every = v;
   } else if (!std::strcmp(key, "prc.max_slots")) {
     if (v < 0) return PRC_EINVAL;
     r->max_slots = v;
@@ -822,6 +850,8 @@ int prc_get_param_int(prc_reactor* r, const char* key, int* v) {
     *v = r->shape.bpc_lu > 0 ? r->shape.bpc_lu : r->occ_lu;
   else if (!std::strcmp(key, "prc.compact_pct"))
     *v = r->shape.compact_pct;
+  else if (!std::strcmp(key, "prc.compact_every"))
+    *v = r->shape.compact_every;
   else if (!std::strcmp(key, "prc.cooperative"))
     *v = r->shape.coop;
   else if (!std::strcmp(key, "prc.coop_tail_slots"))
@@ -1142,6 +1172,7 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
                          {&Shape::bpc_jac, {oj, 2 * oj, 4 * oj, 4 * oj}},
                          {&Shape::bpc_lu, {ol, 2 * ol, 4 * ol, 4 * ol}},
                          {&Shape::compact_pct, {25, 50, 75, 75}},
+                         {&Shape::compact_every, {0, 3, 6, 6}},
                          {&Shape::tail_slots, {0, nt / 64, nt / 16, nt / 4}},
                          {&Shape::bpc_tail, {ot, 4 * ot, 8 * ot, 16 * ot}}};
     for (const Axis& ax : axes) {
@@ -1151,7 +1182,7 @@ int prc_autotune(prc_reactor* r, int64_t ncells, const double* rY, const double*
       for (int c : ax.cand) {
         Shape trial = cur;
         const bool blocks = ax.field != &Shape::iters && ax.field != &Shape::compact_pct &&
-                            ax.field != &Shape::tail_slots;
+                            ax.field != &Shape::tail_slots && ax.field != &Shape::compact_every;
         trial.*ax.field = blocks ? std::min(c, 64) : c;
         if (trial == cur) continue;
         r->shape = trial;

@@ -842,9 +842,23 @@ __global__ void __launch_bounds__(BS) k_newton_g(const Args* __restrict__ ap, in
   }
 }
 
-// Repack live slots (phase != DONE) densely into the other state buffer, lane order preserved.
-__global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap) {
+// The compaction check a graph makes between two iterations (the host makes the same check between
+// replays): compact when fewer than pct % of the slots in use are alive, more than 4096 slots are
+// in use, and no cell of the call waits for a slot (refills stay with the host).
+__global__ void k_compact_gate(const Args* __restrict__ ap, int pct) {
   const Args& a = *ap;
+  if (threadIdx.x == 0) {
+    const long long alive = (long long)a.ctr[CT_STARTED] - (long long)a.ctr[CT_DONE];
+    const long long nact = (long long)a.ctr[CT_NACTIVE];
+    a.ctr[CT_GATE] = (a.ctr[CT_INGRAPH] && nact > 4096 && alive * 100 < nact * pct) ? 1ull : 0ull;
+  }
+}
+
+// Repack live slots (phase != DONE) densely into the other state buffer, lane order preserved.
+// gated: only when the graph's compaction check decided so (k_compact_gate).
+__global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap, int gated) {
+  const Args& a = *ap;
+  if (gated && !a.ctr[CT_GATE]) return;
   const int cur = (int)a.ctr[CT_CUR];
   const StateBuf& src = a.st[cur];
   const StateBuf& dst = a.st[cur ^ 1];
@@ -888,8 +902,9 @@ __global__ void __launch_bounds__(64) k_compact(const Args* __restrict__ ap) {
 }
 
 // Renumber a pending set-up list after a compaction.
-__global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int par) {
+__global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int par, int gated) {
   const Args& a = *ap;
+  if (gated && !a.ctr[CT_GATE]) return;
   const int64_t cnt = (int64_t)a.ctr[CT_SETUP0 + par];
   for (int64_t idx = (int64_t)blockIdx.x * 64 + threadIdx.x; idx < cnt;
        idx += (int64_t)gridDim.x * 64)
@@ -900,9 +915,9 @@ __global__ void __launch_bounds__(64) k_remap(const Args* __restrict__ ap, int p
     a.jlist[par][idx] = a.remap[a.jlist[par][idx]];
 }
 
-__global__ void k_compact_done(const Args* __restrict__ ap) {
+__global__ void k_compact_done(const Args* __restrict__ ap, int gated) {
   const Args& a = *ap;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !(gated && !a.ctr[CT_GATE])) {
     a.ctr[CT_CUR] ^= 1ull;
     a.ctr[CT_NACTIVE] = a.ctr[CT_NEWACTIVE];
     a.ctr[CT_NEWACTIVE] = 0ull;
@@ -910,9 +925,15 @@ __global__ void k_compact_done(const Args* __restrict__ ap) {
 }
 
 // counters are cleared by a kernel rather than a memset node (graph-capturable)
-__global__ void k_init_counters(unsigned long long* c, int64_t nactive) {
+__global__ void k_init_counters(unsigned long long* c, int64_t nactive, int ingraph) {
   if (threadIdx.x < CT_COUNT)
-    c[threadIdx.x] = (threadIdx.x == CT_NACTIVE) ? (unsigned long long)nactive : 0ull;
+    c[threadIdx.x] = (threadIdx.x == CT_NACTIVE)   ? (unsigned long long)nactive
+                     : (threadIdx.x == CT_INGRAPH) ? (unsigned long long)ingraph
+                                                   : 0ull;
+}
+
+__global__ void k_set_counter(unsigned long long* c, int idx, unsigned long long v) {
+  if (threadIdx.x == 0) c[idx] = v;
 }
 
 // ------------------------------------------------------------------ discrete adjoint

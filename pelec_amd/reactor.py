@@ -230,7 +230,8 @@ class Reactor:
         return self
 
     SHAPE_KEYS = ("prc.iters_per_graph", "prc.cooperative", "prc.blocks_newton", "prc.blocks_jac",
-                  "prc.blocks_lu", "prc.compact_pct", "prc.coop_tail_slots", "prc.blocks_tail")
+                  "prc.blocks_lu", "prc.compact_pct", "prc.compact_every", "prc.coop_tail_slots",
+                  "prc.blocks_tail")
 
     def get_param(self, key: str) -> int:
         v = ctypes.c_int(0)

@@ -508,6 +508,38 @@ def oracle_fma(oracle_lib):
     return get
 
 
+def test_graph_compaction_matches_host_compaction(oracles):
+    """prc.compact_every: the captured graph compacts the work queue itself (gate kernel, then the
+    compaction kernels) between iterations. Compaction only moves slots, so with no tail graph the
+    end states are bit-identical to host-only compaction, in graph replays and in the eager,
+    event-timed iterations; both meet the oracle gates (drm19, 16384 cells, FP64 factors)."""
+    wl, n = "drm19_64", 16384
+    W = inputs.WORKLOADS[wl]
+    d = inputs.make_inputs(wl, n)
+    outs = {}
+    for every, kt in ((0, False), (3, False), (3, True)):
+        r = Reactor(W.mech)
+        r.configure(rtol=W.rtol, atol=W.atol, form="batched", lu_fp32=False, kernel_timing=kt)
+        r.set_param("prc.iters_per_graph", 15)
+        r.set_param("prc.compact_pct", 90)
+        r.set_param("prc.compact_every", every)
+        rY, T, rE = to_dev(d["rY"]), to_dev(d["T"]), to_dev(d["rE"])
+        note(f"graph compaction every {every} kernel_timing={kt} n={n}")
+        r.react_flat(rY, T, rE, W.dt)
+        torch.cuda.synchronize()
+        assert r.last_failures() == 0
+        assert r.get_param("prc.compact_every") == every
+        r.close()
+        outs[(every, kt)] = {"rY": rY.cpu().numpy(), "T": T.cpu().numpy()}
+    for k in ((3, False), (3, True)):
+        assert np.array_equal(outs[k]["T"], outs[(0, False)]["T"]), k
+        assert np.array_equal(outs[k]["rY"], outs[(0, False)]["rY"]), k
+    ref = oracles[W.mech].react(d["rY"], d["T"], d["rE"], W.dt, rtol=W.rtol, atol=W.atol)
+    dT, dY, _ = compare(outs[(3, False)], ref)
+    note(f"  max|dT|={dT:.3e} K max|dY|={dY:.3e}")
+    assert dT <= GATE_T and dY <= GATE_Y
+
+
 def test_react_rk64_workload_tolerance_vs_oracle(oracles, oracle_fma):
     """ReactorRK64 on drm19 at the BDF workload's own tolerances (rtol 1e-6, atol 1e-10; dt 1e-6,
     512 cells). At this tolerance a round-off difference flips accept/reject decisions of the
