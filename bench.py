@@ -230,8 +230,9 @@ def time_form(torch, Reactor, wl, d0, form, lu_fp32, steps, warmup, kernel_timin
 
 def run_amr(args, torch, inputs, Reactor):
     """AMR-shaped batch: every box of <mech>_amr (GRI-Mech 3.0 by default) in one prc_react_boxes
-    call of the batched mixed-precision form (launch shape autotuned on a sample of its unmasked
-    cells; the work queue refills its slots when the cells outnumber them), against the plain
+    call of the batched form (FP32 or FP64 Newton-matrix factors, whichever the autotuner times
+    faster on a sample of the unmasked cells, with the launch shape it tuned for that form; the
+    work queue refills its slots when the cells outnumber them), against the plain
     FP64 form called box by box -- on the first --amr-plain-boxes boxes when that is set, a
     stated subset whose per-cell rate is reported; parity against the oracle on a sample of
     unmasked cells; masked cells must come back bit-identical. value = integrated (unmasked)
@@ -263,16 +264,25 @@ def run_amr(args, torch, inputs, Reactor):
 
     act = np.flatnonzero(b["mask"])
     shape = None
+    fp32_forms = [True, False] if args.amr_lu_fp32 < 0 else [bool(args.amr_lu_fp32)]
+    head_fp32 = fp32_forms[0]
+    tuned = {}
     if not args.no_tune:
         k = args.tune_cells if args.tune_cells > 0 else 262144
         samp = act[:: max(1, act.size // k)][:k]
-        tuner = Reactor(wl.mech)
-        tuner.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=True)
         si = torch.tensor(samp, device=dev)
-        shape, _ = tuner.autotune(d0["rY"][:, si].contiguous(), d0["T"][si].contiguous(),
-                                  d0["rE"][si].contiguous(), wl.dt)
-        tuner.close()
-        log(f"autotuned shape {shape}")
+        for fp32 in fp32_forms:
+            tuner = Reactor(wl.mech)
+            tuner.configure(rtol=wl.rtol, atol=wl.atol, form="batched", lu_fp32=fp32)
+            sh, ms = tuner.autotune(d0["rY"][:, si].contiguous(), d0["T"][si].contiguous(),
+                                    d0["rE"][si].contiguous(), wl.dt)
+            tuner.close()
+            tuned[fp32] = (sh, ms)
+            log(f"autotuned shape lu_fp32={fp32}: {sh} ({ms:.1f} ms on {samp.size} sample cells)")
+        head_fp32 = min(tuned, key=lambda f: tuned[f][1])
+        shape = tuned[head_fp32][0]
+    head_name = "batched_fp32lu" if head_fp32 else "batched_fp64lu"
+    log(f"amr headline form: {head_name}")
 
     def timed(form, fp32, steps, warmup, nbox):
         r = Reactor(wl.mech)
@@ -302,7 +312,7 @@ def run_amr(args, torch, inputs, Reactor):
                 "slots": slots}
 
     nb = len(views)
-    head = timed("batched", True, args.steps, args.warmup, nb)
+    head = timed("batched", head_fp32, args.steps, args.warmup, nb)
     npb = nb if args.amr_plain_boxes <= 0 else min(nb, args.amr_plain_boxes)
     plain = timed("plain", False, 1, 0, npb)
     nc_plain = b["boxes"][npb - 1]["first"] + int(np.prod(b["boxes"][npb - 1]["shape"]))
@@ -319,7 +329,7 @@ def run_amr(args, torch, inputs, Reactor):
                                             rtol=wl.rtol, atol=wl.atol, nthreads=host_threads())
     rho = ref["rY"].sum(0)
     par = {}
-    for name, o, lim in (("batched_fp32lu", head, nc), ("plain_fp64", plain, nc_plain)):
+    for name, o, lim in ((head_name, head, nc), ("plain_fp64", plain, nc_plain)):
         sel = idx < lim
         dT = np.abs(o["T"][idx[sel]] - ref["T"][sel])
         dY = (np.abs(o["rY"][:, idx[sel]] - ref["rY"][:, sel]) / rho[sel]).max(0)
@@ -333,11 +343,14 @@ def run_amr(args, torch, inputs, Reactor):
                      "plain FP64 box by box",
            "value": nact / head["t"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["t"] * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 (f32 Newton-matrix factors)" if head_fp32 else "f64", "data": "synthetic",
            "config": {"workload": wl.name, "mechanism": wl.mech, "boxes": len(b["boxes"]),
                       "cells": nc, "unmasked": nact, "edges": list(inputs.AMR_EDGES),
-                      "dt": wl.dt, "rtol": wl.rtol, "atol": wl.atol, "launch_shape": shape,
-                      "work_queue_slots": head["slots"]},
+                      "dt": wl.dt, "rtol": wl.rtol, "atol": wl.atol, "form": head_name,
+                      "launch_shape": shape, "work_queue_slots": head["slots"]},
+           "tuned_sample_ms": {("batched_fp32lu" if f else "batched_fp64lu"): v[1]
+                               for f, v in tuned.items()},
            "plain_fp64": {"value": v_plain, "ms_per_step": plain["t"] * 1e3, "boxes": npb,
                           "unmasked_cells": nact_plain},
            "speedup_vs_plain_fp64": (nact / head["t"]) / v_plain,
@@ -375,6 +388,9 @@ def main():
                          "one call against the plain form box by box")
     ap.add_argument("--amr-boxes", type=int, default=256)
     ap.add_argument("--amr-mech", default="gri30", choices=["gri30", "drm19"])
+    ap.add_argument("--amr-lu-fp32", type=int, default=-1,
+                    help="AMR leg: FP32 (1) or FP64 (0) Newton-matrix factors; -1: the form the "
+                         "autotuner times faster on the sample")
     ap.add_argument("--amr-plain-boxes", type=int, default=0,
                     help="time the plain form on the first K boxes only (0: all)")
     args = ap.parse_args()

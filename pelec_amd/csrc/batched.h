@@ -28,6 +28,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bdf_core.h"
 #include "group_model.h"
 
@@ -217,15 +219,14 @@ __device__ inline void wave_count(unsigned long long* cnt, bool pred) {
 // are not moved: each lane tracks the position its row would occupy in the row-swapping algorithm
 // (pos), so the pivot of step k is the max |a_ik| over positions >= k with ties to the lower
 // position -- the same choice, and the same factors, as the row-swapping algorithm (lu_factor in
-// bdf_cell.h, and the oracle's). The pivot row is broadcast with
-// cross-lane shuffles. On return lane r holds row pos of the factors (unit-lower multipliers left
+// bdf_cell.h, and the oracle's). The pivot search is a DPP butterfly and the pivot row is read
+// by v_readlane (no LDS round trips). On return lane r holds row pos of the factors (unit-lower multipliers left
 // of the diagonal, U right of it, the diagonal inverted) and perm[pos] = r. Returns 0, or k + 1
 // when column k has no nonzero pivot (every lane of the group returns the same value).
 template <int n, typename T, int G>
 __device__ inline int lu_rows(T (&row)[n], int r, int& pos) {
-  static_assert(n <= G && (G & (G - 1)) == 0 && G <= 64, "one row per lane");
+  static_assert(n <= G && (G & (G - 1)) == 0 && G >= 16 && G <= 64, "one row per lane");
   const int lane = threadIdx.x & 63;
-  const int gbase = lane & ~(G - 1);
   pos = r;
   int ier = 0;
 #pragma unroll
@@ -234,17 +235,27 @@ __device__ inline int lu_rows(T (&row)[n], int r, int& pos) {
     T v = (r < n && pos >= k) ? T(fabs(row[k])) : T(-1);
     int kp = pos;
     int src = lane;
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-      const T ov = __shfl_xor(v, off, G);
-      const int op = __shfl_xor(kp, off, G);
-      const int os = __shfl_xor(src, off, G);
-      if (ov > v || (ov == v && op < kp)) {
-        v = ov;
-        kp = op;
-        src = os;
+    // butterfly with decreasing offsets: the key order is total (positions are distinct), so
+    // every lane of the group ends with the same pivot
+    auto step = [&](auto off) {
+      constexpr int o = decltype(off)::value;
+      if constexpr (o < G) {
+        const T ov = xor_partner<G, o>(v);
+        const int op = xor_partner<G, o>(kp);
+        const int os = xor_partner<G, o>(src);
+        if (ov > v || (ov == v && op < kp)) {
+          v = ov;
+          kp = op;
+          src = os;
+        }
       }
-    }
+    };
+    step(std::integral_constant<int, 32>());
+    step(std::integral_constant<int, 16>());
+    step(std::integral_constant<int, 8>());
+    step(std::integral_constant<int, 4>());
+    step(std::integral_constant<int, 2>());
+    step(std::integral_constant<int, 1>());
     if (!(v > T(0))) {
       if (ier == 0) ier = k + 1;
       continue;  // singular: the factors are not used
@@ -255,10 +266,23 @@ __device__ inline int lu_rows(T (&row)[n], int r, int& pos) {
     const int old_pos_pivot = kp;
     if (r < n && pos == k && !is_pivot) pos = old_pos_pivot;
     if (is_pivot) pos = k;
-    // broadcast the pivot row's columns k..n-1
+    // broadcast the pivot row's columns k..n-1: the pivot lane of each group of the wave, read
+    // into scalar registers (it is uniform within a group), then one v_readlane per column and
+    // group
+    int pl[64 / G];
+#pragma unroll
+    for (int g = 0; g < 64 / G; ++g) pl[g] = lane_read(pivot_lane, g * G);
     T prow[n];
 #pragma unroll
-    for (int c = k; c < n; ++c) prow[c] = __shfl(row[c], pivot_lane - gbase, G);
+    for (int c = k; c < n; ++c) {
+      T pv = lane_read(row[c], pl[0]);
+#pragma unroll
+      for (int g = 1; g < 64 / G; ++g) {
+        const T o = lane_read(row[c], pl[g]);
+        pv = lane >= g * G ? o : pv;
+      }
+      prow[c] = pv;
+    }
     const T inv = T(1) / prow[k];
     if (is_pivot) row[k] = inv;
     if (r < n && pos > k) {

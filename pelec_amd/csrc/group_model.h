@@ -18,21 +18,91 @@
 
 namespace pelec {
 
+// Lane exchanges that avoid the LDS crossbar (ds_bpermute, which __shfl uses): v_readlane for a
+// lane index that is the same on every lane of the wave, DPP row rotations inside rows of 16.
+
+// x of lane src. src must be wave-uniform (a constant, a loop constant or a scalar value): the
+// value is read into scalar registers, so every lane receives the same one.
+__device__ inline int lane_read(int x, int src) { return __builtin_amdgcn_readlane(x, src); }
+__device__ inline float lane_read(float x, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+__device__ inline double lane_read(double x, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+  return __hiloint2double(hi, lo);
+}
+
+// x of lane (i -+ k) mod 16 within lane i's row of 16 (DPP row_ror:k).
+template <int k>
+__device__ inline int row_ror(int x) {
+  static_assert(k >= 1 && k <= 15, "row_ror:1..15");
+  return __builtin_amdgcn_mov_dpp(x, 0x120 + k, 0xf, 0xf, false);
+}
+template <int k>
+__device__ inline float row_ror(float x) {
+  return __int_as_float(row_ror<k>(__float_as_int(x)));
+}
+template <int k>
+__device__ inline double row_ror(double x) {
+  return __hiloint2double(row_ror<k>(__double2hiint(x)), row_ror<k>(__double2loint(x)));
+}
+
+// Partner value of the step with offset `off` of an xor butterfly over a group of G >= 16 lanes
+// run with decreasing offsets (G/2, ..., 2, 1). Offsets of 16 and more cross rows and use a
+// shuffle. Below 16, row_ror:off reads lane (i -+ off) mod 16, which is i ^ off with some of the
+// higher bits of the row flipped by the carry; after the steps with the larger offsets every lane
+// holds the same value as each lane that differs from it only in those bits, so the value read is
+// the one lane i ^ off holds: the butterfly combines the same operands in the same order.
+template <int G, int off, typename T>
+__device__ inline T xor_partner(T x) {
+  static_assert(G >= 16 && G <= 64 && off < G, "DPP butterfly within a group of 16..64 lanes");
+  if constexpr (off >= 16)
+    return __shfl_xor(x, off, G);
+  else
+    return row_ror<off>(x);
+}
+
+// x of lane src of the caller's group (src: 0..G-1, wave-uniform), one v_readlane per group of
+// the wave.
+template <int G, typename T>
+__device__ inline T group_read(T x, int src) {
+  T r = lane_read(x, src);
+#pragma unroll
+  for (int g = 1; g < 64 / G; ++g) {
+    const T v = lane_read(x, g * G + src);
+    r = (int)__lane_id() >= g * G ? v : r;
+  }
+  return r;
+}
+
 template <int G>
 struct Group {
-  static_assert(G <= 64 && (G & (G - 1)) == 0, "group of a power-of-two lanes within a wave");
+  static_assert(G <= 64 && G >= 16 && (G & (G - 1)) == 0,
+                "group of a power-of-two lanes (16..64) within a wave");
+  // butterfly sums and maxima: offsets G/2 .. 1 (IEEE addition and fmax are commutative, so
+  // every lane of the group ends with the same bits)
   __device__ static double sum(double x) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, G);
+    if constexpr (G > 32) x += xor_partner<G, 32>(x);
+    if constexpr (G > 16) x += xor_partner<G, 16>(x);
+    x += xor_partner<G, 8>(x);
+    x += xor_partner<G, 4>(x);
+    x += xor_partner<G, 2>(x);
+    x += xor_partner<G, 1>(x);
     return x;
   }
   __device__ static double max(double x) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, G));
+    if constexpr (G > 32) x = fmax(x, xor_partner<G, 32>(x));
+    if constexpr (G > 16) x = fmax(x, xor_partner<G, 16>(x));
+    x = fmax(x, xor_partner<G, 8>(x));
+    x = fmax(x, xor_partner<G, 4>(x));
+    x = fmax(x, xor_partner<G, 2>(x));
+    x = fmax(x, xor_partner<G, 1>(x));
     return x;
   }
-  __device__ static double bcast(double x, int src) { return __shfl(x, src, G); }
-  __device__ static float bcast(float x, int src) { return __shfl(x, src, G); }
+  // x of lane src of the group; src is wave-uniform at every call site (a loop constant)
+  __device__ static double bcast(double x, int src) { return group_read<G>(x, src); }
+  __device__ static float bcast(float x, int src) { return group_read<G>(x, src); }
   // LDS written by one lane and read by another lane of the same wave: LDS operations of a wave
   // are performed in order, so ordering the compiler's accesses is enough.
   __device__ static void sync() {
