@@ -112,6 +112,14 @@ struct Group {
   }
 };
 
+// The cell's LDS block as an LDS-qualified pointer. The model functions below are compiled out of
+// line (rhs, jac, T_from_e), and through a generic pointer they would address the block with flat
+// instructions, which go through the vector memory path and wait on both memory counters. Every
+// caller passes a __shared__ block, so the cast (the low 32 bits of the flat address, the LDS
+// offset) is exact.
+typedef __attribute__((address_space(3))) double lds_double;
+__device__ inline lds_double* lds_ptr(double* p) { return (lds_double*)p; }
+
 // x, made to depend on dep for the compiler: table loads indexed by the result cannot be hoisted
 // above the computation of dep (the tables are constant, so the compiler would otherwise issue
 // every record's loads up front and keep them live across the T(e) iteration).
@@ -187,14 +195,28 @@ struct GroupModel {
   // Not inlined: inlined, its iteration and the kinetics that follow were scheduled together
   // and the register count of the RHS doubled.
   __device__ __noinline__ static double T_from_e(double e, double Yl, int l, double Tguess,
-                                                 double* sh) {
+                                                 double* shg) {
+    lds_double* sh = lds_ptr(shg);
     if (l < N) sh[L_C + l] = Yl;
     Grp::sync();
     if (l < NPE) {
       const int g = l / 12, i = l - 12 * g;
       double acc = 0.0;
-      for (int k = 0; k < N; ++k)
-        if (Mech::T_TMIDG[k] == g) acc += sh[L_C + k] * (&Mech::T_E[k][0][0])[i];
+      // species in batches of CB: a batch's loads are issued together (every entry read is in
+      // range, whatever its temperature group), then added in species order
+#pragma unroll
+      for (int k0 = 0; k0 < N; k0 += CB) {
+        double yv[CB], ev[CB];
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+          const int k = k0 + u < N ? k0 + u : N - 1;
+          yv[u] = sh[L_C + k];
+          ev[u] = (&Mech::T_E[k][0][0])[i];
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u)
+          if (k0 + u < N && Mech::T_TMIDG[k0 + u] == g) acc += yv[u] * ev[u];
+      }
       sh[L_PE + l] = acc;
     }
     Grp::sync();
@@ -208,7 +230,7 @@ struct GroupModel {
   // ---- reactions: lane-level evaluation of reaction j from the tables. Reads C, exp(g/RT), M
   // (and h/RT when JAC) from the cell's LDS block; writes q_j (and its derivatives when JAC).
   template <bool JAC>
-  __device__ static void reaction(int j, double T, double lnT, double invT, double* sh) {
+  __device__ static void reaction(int j, double T, double lnT, double invT, lds_double* sh) {
     const int kind = Mech::T_KIND[j];
     const double lnk = Mech::T_LNA[j] + Mech::T_B[j] * lnT + (-Mech::T_EAR[j]) * invT;
     const double karr = Mech::T_KCONST[j] ? Mech::T_A[j] : exp(lnk);
@@ -321,7 +343,7 @@ struct GroupModel {
       sh[L_DQT + j] = dqdT;
       sh[L_DQM + j] = kind != 0 ? dkdM * net : 0.0;
       // d(prod)/dC of each slot: nu C^(nu-1) times the other slots' factors
-      double* dq = sh + L_DQ + 6 * j;
+      lds_double* dq = sh + L_DQ + 6 * j;
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
         double df = rn[s] == 1 ? 1.0 : (rn[s] == 2 ? 2.0 * cr[s] : 3.0 * cr[s] * cr[s]);
@@ -343,7 +365,7 @@ struct GroupModel {
 
   // Species lane k's thermo into LDS, reactions, and the production rate of species k.
   template <bool JAC>
-  __device__ static double kinetics(double T, double Cl, int l, double* sh, double* eko,
+  __device__ static double kinetics(double T, double Cl, int l, lds_double* sh, double* eko,
                                     double* cvo) {
     const bool sp = l < N;
     const int k = after(sp ? l : 0, T);
@@ -382,15 +404,19 @@ struct GroupModel {
       const int e = Mech::T_CSR_PTR[k + 1];
       for (int p0 = Mech::T_CSR_PTR[k]; p0 < e; p0 += CB) {
         int jj[CB];
-        double nu[CB];
+        double nu[CB], q[CB];
 #pragma unroll
         for (int u = 0; u < CB; ++u) {
           jj[u] = Mech::T_CSR_J[p0 + u];
           nu[u] = Mech::T_CSR_NU[p0 + u];
         }
+        // entries past this species' range name valid reactions (the next species' entries or
+        // the table's padding), so the batch's q_j are read together
+#pragma unroll
+        for (int u = 0; u < CB; ++u) q[u] = sh[L_Q + jj[u]];
 #pragma unroll
         for (int u = 0; u < CB; ++u)
-          if (p0 + u < e) wd += nu[u] * sh[L_Q + jj[u]];
+          if (p0 + u < e) wd += nu[u] * q[u];
       }
     }
     return wd;
@@ -399,13 +425,14 @@ struct GroupModel {
   // ydot for the lane's component (0 on lanes >= n). yl: the lane's state component; srcl: the
   // lane's rhoY source (0 when there is none); sh: the cell's LDS block (RHS_WORDS doubles).
   __device__ static double rhs(double t, double yl, int l, double rhoe0, double rhoesrc,
-                               double srcl, double* sh) {
+                               double srcl, double* shg) {
+    lds_double* sh = lds_ptr(shg);
     const bool sp = l < N;
     const int k = sp ? l : 0;
     const double rho = Grp::sum(sp ? yl : 0.0);
     const double irho = 1.0 / rho;
     const double Tg = Grp::bcast(yl, N);
-    const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, sp ? yl * irho : 0.0, l, Tg, sh);
+    const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, sp ? yl * irho : 0.0, l, Tg, shg);
     double ek = 0.0, cvk = 0.0;
     const double wd = kinetics<false>(T, yl * Mech::IW[k], l, sh, &ek, &cvk);
     const double fk = sp ? Mech::W[k] * wd + srcl : 0.0;
@@ -418,18 +445,19 @@ struct GroupModel {
   // Exact Jacobian of rhs() (row-major n x n, the layout of CellModel::jac) written to Jg by the
   // group; sh: the cell's LDS block (JAC_WORDS doubles).
   __device__ static void jac(double t, double yl, int l, double rhoe0, double rhoesrc, double srcl,
-                             double* sh, double* Jg) {
+                             double* shg, double* Jg) {
+    lds_double* sh = lds_ptr(shg);
     const bool sp = l < N;
     const int k = sp ? l : 0;
     const double rho = Grp::sum(sp ? yl : 0.0);
     const double irho = 1.0 / rho;
     const double Tg = Grp::bcast(yl, N);
-    const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, sp ? yl * irho : 0.0, l, Tg, sh);
+    const double T = T_from_e((rhoe0 + t * rhoesrc) * irho, sp ? yl * irho : 0.0, l, Tg, shg);
     double ek = 0.0, cvk = 0.0;
     const double Cl = yl * Mech::IW[k];
     // kinetics<true> also leaves the per-reaction derivatives in LDS
     const double wd = kinetics<true>(T, Cl, l, sh, &ek, &cvk);
-    double* row = sh + L_J + k * JS;
+    lds_double* row = sh + L_J + k * JS;
     double dwdT = 0.0;
     if (sp) {
       double tb[NS];
@@ -453,7 +481,7 @@ struct GroupModel {
           if (p0 + u >= e) break;
           const int j = jj[u];
           dwdT += nu[u] * sh[L_DQT + j];
-          const double* dq = sh + L_DQ + 6 * j;
+          const lds_double* dq = sh + L_DQ + 6 * j;
 #pragma unroll
           for (int q = 0; q < 6; ++q)
             if (sl[u][q] >= 0) row[sl[u][q]] += nu[u] * dq[q];
